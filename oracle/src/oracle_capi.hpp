@@ -1,0 +1,281 @@
+// C-ABI of the oracle for one application (same conventions as include/ppf.h, with
+// every array on deal.II's full DoF set).  TEST INFRASTRUCTURE.
+//
+// An oracle translation unit defines its App description, then
+//     #define ORACLE_APP App
+//     #include "oracle_capi.hpp"
+#pragma once
+
+#include "oracle.hpp"
+
+#include <cstring>
+#include <exception>
+#include <string>
+
+namespace oracle
+{
+  struct Handle
+  {
+    explicit Handle(const std::string &prm)
+      : model(prm)
+    {}
+    Model<ORACLE_APP> model;
+    std::string       error;
+  };
+
+  inline std::string &
+  create_error()
+  {
+    static thread_local std::string e;
+    return e;
+  }
+
+  template <class F>
+  int
+  guard(void *h, F &&f)
+  {
+    auto *c = static_cast<Handle *>(h);
+    if (!c)
+      return -1;
+    try
+      {
+        f(c->model);
+        c->error.clear();
+        return 0;
+      }
+    catch (const std::exception &e)
+      {
+        c->error = e.what();
+        return -1;
+      }
+  }
+} // namespace oracle
+
+extern "C" {
+
+int
+oracle_create(const char *prm, void **out)
+{
+  try
+    {
+      *out = new oracle::Handle(prm);
+      return 0;
+    }
+  catch (const std::exception &e)
+    {
+      oracle::create_error() = e.what();
+      *out                   = nullptr;
+      return -1;
+    }
+}
+
+void
+oracle_destroy(void *h)
+{
+  delete static_cast<oracle::Handle *>(h);
+}
+
+const char *
+oracle_last_error(void *h)
+{
+  return h ? static_cast<oracle::Handle *>(h)->error.c_str() : oracle::create_error().c_str();
+}
+
+int
+oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#endif
+  return 0;
+}
+
+int
+oracle_info(void *h, long long *ndofs, int *nfields, int *nblocks)
+{
+  return oracle::guard(h, [&](auto &m) {
+    *ndofs   = m.top.ndofs;
+    *nfields = int(m.fields.size());
+    *nblocks = int(m.blocks.size());
+  });
+}
+
+int
+oracle_set_option(void *h, const char *key, const char *value)
+{
+  return oracle::guard(h, [&](auto &m) {
+    const std::string k = key, v = value;
+    for (auto &b : m.blocks)
+      {
+        auto &lp = b.linear_solver_parameters;
+        if (k == "linear_tolerance")
+          lp.tolerance = std::stod(v);
+        else if (k == "linear_max_iterations")
+          lp.max_iterations = unsigned(std::stoul(v));
+        else if (k == "mg_coarse_solver")
+          lp.coarse_solver =
+            v == "cg" ? prismspf::CoarseSolver::CG : prismspf::CoarseSolver::Smoother;
+        else if (k == "mg_level_precision")
+          ; // the oracle is FP64 throughout
+        else
+          throw std::invalid_argument("unknown option " + k);
+      }
+  });
+}
+
+int
+oracle_set_field(void *h, int f, int age, const double *v, long long n)
+{
+  return oracle::guard(h, [&](auto &m) {
+    if (n != m.top.ndofs || f < 0 || f >= int(m.storage.size()) || age < 0 ||
+        age >= int(m.storage[f].size()))
+      throw std::invalid_argument("oracle_set_field: bad field, age or size");
+    std::copy(v, v + n, m.storage[f][age].begin());
+  });
+}
+
+int
+oracle_get_field(void *h, int f, int age, double *v, long long n)
+{
+  return oracle::guard(h, [&](auto &m) {
+    if (n != m.top.ndofs || f < 0 || f >= int(m.storage.size()) || age < 0 ||
+        age >= int(m.storage[f].size()))
+      throw std::invalid_argument("oracle_get_field: bad field, age or size");
+    std::copy(m.storage[f][age].begin(), m.storage[f][age].end(), v);
+  });
+}
+
+int
+oracle_set_time(void *h, unsigned increment, double time)
+{
+  return oracle::guard(h, [&](auto &m) {
+    m.timer.set_increment(increment);
+    m.timer.set_time(time);
+  });
+}
+
+int
+oracle_apply_operator(void *h, int block_id, int which, int scaling, const double *src,
+                      double *dst)
+{
+  return oracle::guard(h, [&](auto &m) {
+    m.apply_top(m.block_index(block_id), which != 0, src, dst, scaling, which == 2);
+  });
+}
+
+int
+oracle_inverse_diagonal(void *h, int block_id, int scaling, double *dst)
+{
+  return oracle::guard(h, [&](auto &m) {
+    const int b = m.block_index(block_id);
+    auto      d = m.inverse_diagonal(b, m.top, m.cons, m.jxw, m.storage, scaling);
+    std::copy(d.begin(), d.end(), dst);
+  });
+}
+
+int
+oracle_solve_block(void *h, int block_id)
+{
+  return oracle::guard(h, [&](auto &m) { m.solve_block(m.block_index(block_id)); });
+}
+
+int
+oracle_rotate_old(void *h, int block_id)
+{
+  return oracle::guard(h, [&](auto &m) { m.rotate(m.block_index(block_id)); });
+}
+
+int
+oracle_solve_increment(void *h, int *status)
+{
+  return oracle::guard(h, [&](auto &m) {
+    *status = m.solve_increment();
+    m.timer.increment();
+  });
+}
+
+int
+oracle_run(void *h, unsigned n, int *status)
+{
+  return oracle::guard(h, [&](auto &m) {
+    int s = 0;
+    while (m.timer.get_increment() <= n && s == 0)
+      {
+        s = m.solve_increment();
+        m.timer.increment();
+      }
+    *status = s;
+  });
+}
+
+int
+oracle_linear_solve(void *h, int block_id, int *its, double *r0, double *r, double *tol,
+                    int *converged)
+{
+  return oracle::guard(h, [&](auto &m) {
+    auto st    = m.solve_linear(m.block_index(block_id));
+    *its       = st.iterations;
+    *r0        = st.r0;
+    *r         = st.r;
+    *tol       = st.tol;
+    *converged = st.converged ? 1 : 0;
+  });
+}
+
+int
+oracle_field_norms(void *h, int f, int age, double *l2, double *integral)
+{
+  return oracle::guard(h, [&](auto &m) {
+    auto r    = m.norms(f, age);
+    *l2       = r.first;
+    *integral = r.second;
+  });
+}
+
+const char *
+oracle_output_log(void *h)
+{
+  return static_cast<oracle::Handle *>(h)->model.log.c_str();
+}
+
+int
+oracle_lumped_mass(void *h, double *out)
+{
+  return oracle::guard(h, [&](auto &m) { std::copy(m.jxw.begin(), m.jxw.end(), out); });
+}
+
+int
+oracle_adjoint_forward(void *h, const double *phi0, double L, double kappa, unsigned n,
+                       double *out)
+{
+  return oracle::guard(h, [&](auto &m) {
+    using A = typename std::decay_t<decltype(m)>::AppType;
+    if constexpr (oracle::has_energy<A>::value)
+      {
+        std::vector<double> p(phi0, phi0 + m.top.ndofs);
+        auto r = m.template ac_forward<typename A::FreeEnergy>(p, L, kappa, n);
+        std::copy(r.begin(), r.end(), out);
+      }
+    else
+      throw std::invalid_argument("no free energy declared");
+  });
+}
+
+int
+oracle_adjoint_vjp(void *h, const double *phi0, const double *lam, double L, double kappa,
+                   unsigned n, double *dL, double *dk, double *dphi0)
+{
+  return oracle::guard(h, [&](auto &m) {
+    using A = typename std::decay_t<decltype(m)>::AppType;
+    if constexpr (oracle::has_energy<A>::value)
+      {
+        std::vector<double> p(phi0, phi0 + m.top.ndofs), l(lam, lam + m.top.ndofs), g;
+        m.template ac_vjp<typename A::FreeEnergy>(p, l, L, kappa, n, *dL, *dk, g);
+        std::copy(g.begin(), g.end(), dphi0);
+      }
+    else
+      throw std::invalid_argument("no free energy declared");
+  });
+}
+
+} // extern "C"

@@ -1,0 +1,35 @@
+// Coupled Allen-Cahn / Cahn-Hilliard precipitate model, 3-D, Q1: library of the
+// benchmark configuration "Coupled Allen-Cahn/Cahn-Hilliard precipitate 3D"
+// (192^3 cells, fields c, mu, eta).
+#include "../../csrc/app_prelude.hpp"
+
+#pragma clang force_cuda_host_device begin
+#include "custom_pde.h"
+#pragma clang force_cuda_host_device end
+
+#include "app_fields.h"
+
+struct Precipitate3D
+{
+  static constexpr int dim      = 3;
+  static constexpr int degree   = 1;
+  static constexpr int n_fields = 3;
+  template <typename number>
+  using PDE      = prismspf::CustomPDE<dim, degree, number>;
+  using BlockIds = std::integer_sequence<int, 0, 1, 2>;
+
+  static const char *
+  name()
+  {
+    return "precipitate_3d_q1";
+  }
+
+  static void
+  define(std::vector<prismspf::FieldAttributes> &fields, std::vector<prismspf::SolveBlock> &blocks)
+  {
+    precipitate_define(fields, blocks);
+  }
+};
+
+#define PPF_APP Precipitate3D
+#include "../../csrc/capi_impl.hpp"

@@ -1,0 +1,488 @@
+// Implementation of the C-ABI declared in include/ppf.h for one application.
+//
+// An application translation unit defines its `App` description, then
+//     #define PPF_APP App
+//     #include "capi_impl.hpp"
+// which instantiates the runtime and kernels for that application and exports the
+// `ppf_*` symbols.  Exceptions never cross the boundary: each entry point catches,
+// records the message on the context, and returns a status code.
+#pragma once
+
+#ifndef PPF_APP
+#  error "define PPF_APP to the application description before including capi_impl.hpp"
+#endif
+
+#include "adjoint.hpp"
+#include "runtime.hpp"
+
+#include <ppf.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+
+namespace ppf
+{
+  template <class A, class = void>
+  struct has_free_energy : std::false_type
+  {};
+  template <class A>
+  struct has_free_energy<A, std::void_t<typename A::FreeEnergy>> : std::true_type
+  {};
+
+  struct Context
+  {
+    explicit Context(const std::string &prm)
+      : rt(prm)
+    {}
+    Runtime<PPF_APP> rt;
+    std::string      error;
+    bool             synchronize = true;
+  };
+
+  inline std::string &
+  global_error()
+  {
+    static thread_local std::string e;
+    return e;
+  }
+
+  template <class F>
+  int
+  guarded(ppf_ctx *ctx, F &&f)
+  {
+    auto *c = reinterpret_cast<Context *>(ctx);
+    if (!c)
+      {
+        global_error() = "null context";
+        return PPF_ERR_INVALID;
+      }
+    try
+      {
+        f(*c);
+        c->error.clear();
+        return PPF_OK;
+      }
+    catch (const Error &e)
+      {
+        c->error = e.what();
+        return e.code;
+      }
+    catch (const std::exception &e)
+      {
+        c->error = e.what();
+        return PPF_ERR_INVALID;
+      }
+    catch (...)
+      {
+        c->error = "unknown error";
+        return PPF_ERR_INTERNAL;
+      }
+  }
+
+  /// Adjoint entry points for applications that declare a free energy (the discarded
+  /// branch must depend on a template parameter).
+  template <class A>
+  void
+  adjoint_forward(Context &c, const double *phi0, double L, double kappa, unsigned n,
+                  int precision, double *phi_final)
+  {
+    if constexpr (has_free_energy<A>::value)
+      {
+        auto &rt = c.rt;
+        AllenCahnAdjoint<A::dim, A::degree, typename A::FreeEnergy> adj(rt.fine, rt.stream);
+        adj.forward(phi0, phi_final, rt.timer.get_timestep(), L, kappa, n, precision);
+        rt.sync();
+      }
+    else
+      throw Error(status_unsupported, "this application declares no free energy for the adjoint");
+  }
+
+  template <class A>
+  void
+  adjoint_vjp(Context &c, const double *phi0, const double *lam, double L, double kappa,
+              unsigned n, unsigned every, int precision, double &dL, double &dk, double *dphi0)
+  {
+    if constexpr (has_free_energy<A>::value)
+      {
+        auto &rt = c.rt;
+        AllenCahnAdjoint<A::dim, A::degree, typename A::FreeEnergy> adj(rt.fine, rt.stream);
+        adj.vjp(phi0, lam, rt.timer.get_timestep(), L, kappa, n, every, precision, dL, dk, dphi0);
+      }
+    else
+      throw Error(status_unsupported, "this application declares no free energy for the adjoint");
+  }
+} // namespace ppf
+
+extern "C" {
+
+const char *
+ppf_app_name(void)
+{
+  return PPF_APP::name();
+}
+
+int
+ppf_create(const char *prm_text, ppf_ctx **out)
+{
+  if (!prm_text || !out)
+    {
+      ppf::global_error() = "ppf_create: null argument";
+      return PPF_ERR_INVALID;
+    }
+  *out = nullptr;
+  try
+    {
+      *out = reinterpret_cast<ppf_ctx *>(new ppf::Context(prm_text));
+      return PPF_OK;
+    }
+  catch (const ppf::Error &e)
+    {
+      ppf::global_error() = e.what();
+      return e.code;
+    }
+  catch (const std::exception &e)
+    {
+      ppf::global_error() = e.what();
+      return PPF_ERR_INVALID;
+    }
+}
+
+void
+ppf_destroy(ppf_ctx *ctx)
+{
+  delete reinterpret_cast<ppf::Context *>(ctx);
+}
+
+const char *
+ppf_last_error(const ppf_ctx *ctx)
+{
+  if (!ctx)
+    return ppf::global_error().c_str();
+  return reinterpret_cast<const ppf::Context *>(ctx)->error.c_str();
+}
+
+int
+ppf_get_info(ppf_ctx *ctx, ppf_info *info)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    if (!info)
+      throw ppf::Error(ppf::status_invalid, "null info");
+    const auto &g = c.rt.fine;
+    std::memset(info, 0, sizeof(*info));
+    info->dim    = g.dim;
+    info->degree = g.degree;
+    for (int d = 0; d < 3; ++d)
+      {
+        info->cells[d]    = g.cells[d];
+        info->nodes[d]    = g.nodes[d];
+        info->periodic[d] = g.periodic[d];
+        info->lower[d]    = g.lower[d];
+        info->h[d]        = g.h[d];
+      }
+    info->n_nodes      = g.n_nodes;
+    info->n_dofs       = c.rt.full_size(g);
+    info->n_fields     = int(c.rt.fields.size());
+    info->n_blocks     = int(c.rt.blocks.size());
+    info->dt           = c.rt.timer.get_timestep();
+    info->time         = c.rt.timer.get_time();
+    info->increment    = c.rt.timer.get_increment();
+    info->n_increments = c.rt.inputs.temporal_discretization.n_increments;
+  });
+}
+
+int
+ppf_field_name(ppf_ctx *ctx, int field, char *buf, int len)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    if (field < 0 || field >= int(c.rt.fields.size()) || !buf || len <= 0)
+      throw ppf::Error(ppf::status_invalid, "bad field or buffer");
+    std::strncpy(buf, c.rt.fields[field].name.c_str(), size_t(len) - 1);
+    buf[len - 1] = '\0';
+  });
+}
+
+int
+ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    const std::string k = key ? key : "", v = value ? value : "";
+    if (k == "operator_path")
+      {
+        if (v == "auto")
+          c.rt.path = ppf::OperatorPath::Auto;
+        else if (v == "generic")
+          c.rt.path = ppf::OperatorPath::Generic;
+        else if (v == "split")
+          c.rt.path = ppf::OperatorPath::Split;
+        else
+          throw ppf::Error(ppf::status_invalid, "operator_path must be auto|generic|split");
+      }
+    else if (k == "synchronize")
+      c.synchronize = v != "0";
+    else if (k == "mg_level_precision")
+      {
+        for (auto &b : c.rt.blocks)
+          b.linear_solver_parameters.level_precision =
+            (v == "single" || v == "float" || v == "fp32") ? prismspf::LevelPrecision::Single
+                                                           : prismspf::LevelPrecision::Double;
+      }
+    else if (k == "mg_coarse_solver")
+      {
+        for (auto &b : c.rt.blocks)
+          b.linear_solver_parameters.coarse_solver =
+            v == "cg" ? prismspf::CoarseSolver::CG : prismspf::CoarseSolver::Smoother;
+      }
+    else if (k == "linear_tolerance")
+      {
+        for (auto &b : c.rt.blocks)
+          b.linear_solver_parameters.tolerance = std::stod(v);
+      }
+    else if (k == "linear_max_iterations")
+      {
+        for (auto &b : c.rt.blocks)
+          b.linear_solver_parameters.max_iterations = unsigned(std::stoul(v));
+      }
+    else
+      throw ppf::Error(ppf::status_invalid, "unknown option " + k);
+  });
+}
+
+int
+ppf_set_field(ppf_ctx *ctx, int field, int age, const double *host, long long n)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) { c.rt.set_field(field, age, host, n); });
+}
+
+int
+ppf_get_field(ppf_ctx *ctx, int field, int age, double *host, long long n)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) { c.rt.get_field(field, age, host, n); });
+}
+
+int
+ppf_field_device_ptr(ppf_ctx *ctx, int field, int age, void **ptr)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) { *ptr = c.rt.field_ptr(field, age); });
+}
+
+int
+ppf_run(ppf_ctx *ctx, unsigned int n_increments, int *exit_status)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    const int s = c.rt.run(n_increments);
+    if (exit_status)
+      *exit_status = s;
+  });
+}
+
+int
+ppf_solve_increment(ppf_ctx *ctx, int *exit_status)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    const int s = c.rt.solve_increment();
+    c.rt.timer.increment();
+    c.rt.sync();
+    if (exit_status)
+      *exit_status = s;
+  });
+}
+
+int
+ppf_solve_block(ppf_ctx *ctx, int block_id)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    c.rt.solve_block(c.rt.block_index(block_id));
+    c.rt.sync();
+  });
+}
+
+int
+ppf_rotate_old(ppf_ctx *ctx, int block_id)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) { c.rt.rotate(c.rt.block_index(block_id)); });
+}
+
+int
+ppf_set_time(ppf_ctx *ctx, unsigned int increment, double time)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    c.rt.timer.set_increment(increment);
+    c.rt.timer.set_time(time);
+  });
+}
+
+int
+ppf_apply_operator(ppf_ctx *ctx, int block_id, int which, int scaling, int precision,
+                   const void *d_src, void *d_dst)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    auto     &rt  = c.rt;
+    const int b   = rt.block_index(block_id);
+    const int nbf = int(rt.blocks[b].field_indices.size());
+    if (which < PPF_OP_RHS || which > PPF_OP_LHS_PLAIN || scaling < 0 || scaling > 2)
+      throw ppf::Error(ppf::status_invalid, "bad operator selector or scaling");
+    if (!d_dst || (which != PPF_OP_RHS && !d_src))
+      throw ppf::Error(ppf::status_invalid, "null device array");
+    auto run = [&](auto tag) {
+      using T    = decltype(tag);
+      auto lf    = typename ppf::Runtime<PPF_APP>::template LevelFields<T> {};
+      lf.grid    = &rt.fine;
+      std::vector<ppf::DeviceBuffer<T>> shadow;
+      if constexpr (std::is_same_v<T, double>)
+        lf = rt.fine_fields();
+      else
+        {
+          // FP32 copies of the non-operand fields.
+          for (std::size_t f = 0; f < rt.storage.size(); ++f)
+            for (std::size_t a = 0; a < rt.storage[f].size(); ++a)
+              {
+                shadow.emplace_back(rt.fine.n_nodes);
+                rt.convert(shadow.back().get(), rt.storage[f][a].get(), rt.fine.n_nodes);
+                lf.ptr[f][a + 1] = shadow.back().get();
+              }
+        }
+      std::vector<const T *> src;
+      std::vector<T *>       dst;
+      for (int k = 0; k < nbf; ++k)
+        {
+          if (d_src)
+            src.push_back(static_cast<const T *>(d_src) + k * rt.fine.n_nodes);
+          dst.push_back(static_cast<T *>(d_dst) + k * rt.fine.n_nodes);
+        }
+      if (which == PPF_OP_RHS)
+        src.clear();
+      rt.template apply_operator<T>(b, which != PPF_OP_RHS, lf, src, dst, scaling,
+                                    ppf::dirichlet_zero, which == PPF_OP_LHS_PLAIN);
+      if (c.synchronize || !shadow.empty())
+        rt.sync();
+    };
+    if (precision == PPF_FP32)
+      run(float {});
+    else
+      run(double {});
+  });
+}
+
+int
+ppf_inverse_diagonal(ppf_ctx *ctx, int block_id, int scaling, int precision, void *d_dst)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    auto     &rt = c.rt;
+    const int b  = rt.block_index(block_id);
+    if (precision == PPF_FP32)
+      {
+        auto lf = typename ppf::Runtime<PPF_APP>::template LevelFields<float> {};
+        lf.grid = &rt.fine;
+        std::vector<ppf::DeviceBuffer<float>> shadow;
+        for (std::size_t f = 0; f < rt.storage.size(); ++f)
+          for (std::size_t a = 0; a < rt.storage[f].size(); ++a)
+            {
+              shadow.emplace_back(rt.fine.n_nodes);
+              rt.convert(shadow.back().get(), rt.storage[f][a].get(), rt.fine.n_nodes);
+              lf.ptr[f][a + 1] = shadow.back().get();
+            }
+        rt.template inverse_diagonal<float>(b, lf, scaling, static_cast<float *>(d_dst));
+      }
+    else
+      rt.template inverse_diagonal<double>(b, rt.fine_fields(), scaling,
+                                           static_cast<double *>(d_dst));
+    rt.sync();
+  });
+}
+
+int
+ppf_linear_solve(ppf_ctx *ctx, int block_id, ppf_solve_stats *stats)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    const auto st = c.rt.solve_linear(c.rt.block_index(block_id));
+    if (stats)
+      *stats = {st.iterations, st.initial_residual, st.final_residual, st.tolerance,
+                st.converged};
+  });
+}
+
+int
+ppf_last_solve_stats(ppf_ctx *ctx, int block_id, ppf_solve_stats *stats)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    const auto &st = c.rt.linear_state(c.rt.block_index(block_id)).stats;
+    if (stats)
+      *stats = {st.iterations, st.initial_residual, st.final_residual, st.tolerance,
+                st.converged};
+  });
+}
+
+int
+ppf_field_norms(ppf_ctx *ctx, int field, int age, double *l2, double *integral)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    const auto r = c.rt.norms(field, age);
+    if (l2)
+      *l2 = r.first;
+    if (integral)
+      *integral = r.second;
+  });
+}
+
+const char *
+ppf_output_log(ppf_ctx *ctx)
+{
+  if (!ctx)
+    return "";
+  return reinterpret_cast<ppf::Context *>(ctx)->rt.log.c_str();
+}
+
+void *
+ppf_stream(ppf_ctx *ctx)
+{
+  if (!ctx)
+    return nullptr;
+  return reinterpret_cast<ppf::Context *>(ctx)->rt.stream;
+}
+
+int
+ppf_synchronize(ppf_ctx *ctx)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) { c.rt.sync(); });
+}
+
+int
+ppf_adjoint_forward(ppf_ctx *ctx, const double *d_phi0, double L, double kappa,
+                    unsigned int n_steps, int precision, double *d_phi_final)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    ppf::adjoint_forward<PPF_APP>(c, d_phi0, L, kappa, n_steps, precision, d_phi_final);
+  });
+}
+
+int
+ppf_adjoint_vjp(ppf_ctx *ctx, const double *d_phi0, const double *d_lam_final, double L,
+                double kappa, unsigned int n_steps, unsigned int checkpoint_every,
+                int precision, double *dL, double *dkappa, double *d_dphi0)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    double gl = 0.0, gk = 0.0;
+    ppf::adjoint_vjp<PPF_APP>(c, d_phi0, d_lam_final, L, kappa, n_steps, checkpoint_every,
+                              precision, gl, gk, d_dphi0);
+    if (dL)
+      *dL = gl;
+    if (dkappa)
+      *dkappa = gk;
+  });
+}
+
+int
+ppf_lumped_mass(ppf_ctx *ctx, double *d_mass)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    auto &rt = c.rt;
+    hipLaunchKernelGGL((ppf::lumped_mass_kernel<PPF_APP::dim, PPF_APP::degree>),
+                       dim3(ppf::grid_for(rt.fine.n_nodes, 256)), dim3(256), 0, rt.stream, d_mass,
+                       rt.fine);
+    rt.sync();
+  });
+}
+
+} // extern "C"

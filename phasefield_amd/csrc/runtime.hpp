@@ -1,0 +1,1481 @@
+// Host runtime of one application on one GPU: field storage, operator application,
+// the explicit and linear solvers (CG with Chebyshev or geometric-multigrid
+// preconditioning, optional FP32 levels), and the increment loop.
+//
+// It is the device counterpart of the reference's solver layer
+// (reference: include/prismspf/solvers/{explicit_solver,linear_solver,solver_base}.h,
+// src/solvers/mf_operator.cc, src/core/{problem,group_solution_handler}.cc) and keeps
+// its ordering and constraint semantics; everything heavy runs as HIP kernels on the
+// runtime's stream.  `App` supplies the dimension, degree, the equation class and the
+// field/solve-block declarations (the body of a reference `main.cc`).
+#pragma once
+
+#include "device.hpp"
+#include "grid.hpp"
+#include "operator_kernels.hpp"
+#include "operator_plan.hpp"
+#include "vector_kernels.hpp"
+
+#include <prismspf/core/field_attributes.h>
+#include <prismspf/core/phase_field_tools.h>
+#include <prismspf/core/simulation_timer.h>
+#include <prismspf/core/solve_block.h>
+#include <prismspf/user_inputs/user_input_parameters.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace ppf
+{
+  enum class OperatorPath : int
+  {
+    Auto    = 0,
+    Generic = 1,
+    Split   = 2
+  };
+
+  /// Statistics of the last linear solve of a block.
+  struct SolveStats
+  {
+    int    iterations       = 0;
+    double initial_residual = 0.0;
+    double final_residual   = 0.0;
+    double tolerance        = 0.0;
+    int    converged        = 0;
+  };
+
+  template <int... ids, class F>
+  void
+  dispatch_block_id(std::integer_sequence<int, ids...>, int id, F &&f)
+  {
+    const bool found = ((id == ids ? (f(std::integral_constant<int, ids> {}), true) : false) ||
+                        ...);
+    if (!found)
+      throw Error(status_invalid, "solve block id " + std::to_string(id) +
+                                    " is not compiled into this application library");
+  }
+
+  /// Tridiagonal symmetric eigenvalues by bisection (Sturm sequences); used for the
+  /// Lanczos estimate of the Chebyshev eigenvalue range.
+  inline double
+  tridiagonal_max_eigenvalue(const std::vector<double> &a, const std::vector<double> &b)
+  {
+    const int n = int(a.size());
+    if (n == 0)
+      return 0.0;
+    double lo = a[0], hi = a[0];
+    for (int i = 0; i < n; ++i)
+      {
+        const double r = (i > 0 ? std::abs(b[i - 1]) : 0.0) + (i + 1 < n ? std::abs(b[i]) : 0.0);
+        lo             = std::min(lo, a[i] - r);
+        hi             = std::max(hi, a[i] + r);
+      }
+    auto count_below = [&](double x) {
+      int    c = 0;
+      double q = a[0] - x;
+      if (q < 0)
+        ++c;
+      for (int i = 1; i < n; ++i)
+        {
+          const double denom = q == 0.0 ? 1e-300 : q;
+          q                  = a[i] - x - b[i - 1] * b[i - 1] / denom;
+          if (q < 0)
+            ++c;
+        }
+      return c;
+    };
+    for (int it = 0; it < 200; ++it)
+      {
+        const double mid = 0.5 * (lo + hi);
+        if (count_below(mid) >= n)
+          hi = mid;
+        else
+          lo = mid;
+      }
+    return hi;
+  }
+
+  template <class App>
+  class Runtime
+  {
+  public:
+    static constexpr int dim    = App::dim;
+    static constexpr int degree = App::degree;
+    template <typename number>
+    using PDE = typename App::template PDE<number>;
+
+    explicit Runtime(const std::string &prm_text)
+      : inputs(prismspf::UserInputParameters<dim>::from_text(prm_text))
+    {
+      App::define(fields, blocks);
+      prismspf::validate_solve_blocks(blocks, fields);
+      if (int(fields.size()) != App::n_fields)
+        throw Error(status_internal, "App::n_fields does not match the declared fields");
+      for (const auto &fa : fields)
+        if (fa.field_type != prismspf::Scalar)
+          throw Error(status_unsupported, "field " + fa.name +
+                                            ": only scalar fields are supported on the "
+                                            "MI355X path");
+      if (fields.size() > PPF_MAX_FIELDS)
+        throw Error(status_unsupported, "more than PPF_MAX_FIELDS fields");
+
+      // Parameter-file overrides of the declared solver settings and boundary
+      // conditions (reference problem.cc:152-224).
+      for (auto &b : blocks)
+        {
+          const auto it = inputs.linear_solve_parameters.find(b.id);
+          if (it != inputs.linear_solve_parameters.end())
+            b.linear_solver_parameters = it->second;
+        }
+      for (auto &fa : fields)
+        {
+          const auto it = inputs.boundary_parameters.find(fa.name);
+          if (it != inputs.boundary_parameters.end())
+            fa.boundary_conditions = it->second;
+        }
+
+      pde_d = std::make_unique<PDE<double>>(inputs, tools);
+      pde_f = std::make_unique<PDE<float>>(inputs, tools);
+      timer = prismspf::SimulationTimer(inputs.temporal_discretization.dt,
+                                        inputs.temporal_discretization.initial_time);
+
+      fine = make_grid(inputs.spatial_discretization.global_refinement);
+      PPF_HIP_CHECK(hipStreamCreate(&stream));
+      reduce_buf.resize(2 * reduce_blocks + 8);
+      scalar_buf.resize(8);
+      setup_storage();
+      compute_dirichlet_values(fine, dirichlet_values);
+    }
+
+    ~Runtime()
+    {
+      if (stream)
+        (void) hipStreamDestroy(stream);
+    }
+
+    Runtime(const Runtime &) = delete;
+    Runtime &operator=(const Runtime &) = delete;
+
+    // ---------------------------------------------------------------- geometry
+    GridDesc
+    make_grid(unsigned int refinement) const
+    {
+      const auto &mesh = inputs.spatial_discretization.rectangular_mesh;
+      GridDesc    g {};
+      g.dim     = dim;
+      g.degree  = degree;
+      g.n_nodes = 1;
+      g.n_cells = 1;
+      for (int d = 0; d < 3; ++d)
+        {
+          if (d < dim)
+            {
+              g.cells[d]    = int(mesh.subdivisions[d] << refinement);
+              g.periodic[d] = mesh.periodic_directions.count(d) ? 1 : 0;
+              g.nodes[d]    = degree * g.cells[d] + (g.periodic[d] ? 0 : 1);
+              g.lower[d]    = mesh.lower_bound[d];
+              g.h[d]        = (mesh.size[d] - mesh.lower_bound[d]) / g.cells[d];
+              if (!(g.h[d] > 0.0))
+                throw Error(status_invalid, "mesh size must exceed the lower bound");
+            }
+          else
+            {
+              g.cells[d]    = 1;
+              g.nodes[d]    = 1;
+              g.periodic[d] = 0;
+              g.lower[d]    = 0.0;
+              g.h[d]        = 1.0;
+            }
+          g.n_nodes *= g.nodes[d];
+          g.n_cells *= g.cells[d];
+        }
+      return g;
+    }
+
+    /// deal.II's node count with periodic images, (p n + 1)^d.
+    int64_t
+    full_size(const GridDesc &g) const
+    {
+      int64_t n = 1;
+      for (int d = 0; d < dim; ++d)
+        n *= int64_t(degree) * g.cells[d] + 1;
+      return n;
+    }
+
+    /// Dirichlet face mask of a field (faces in periodic directions never count).
+    unsigned int
+    dirichlet_mask(int f) const
+    {
+      unsigned int m = 0;
+      for (const auto &[comp, conds] : fields[f].boundary_conditions.component_constraints)
+        for (const auto &[bid, c] : conds)
+          if (c == prismspf::Condition::Dirichlet && bid < 2u * dim &&
+              !fine.periodic[bid / 2])
+            m |= 1u << bid;
+      return m;
+    }
+
+    // ---------------------------------------------------------------- storage
+    void
+    setup_storage()
+    {
+      // Oldest age read of each field, over every block (reference
+      // dependency_extents.h:75-133 computes it per block group; the union is the same
+      // for rotation because each field belongs to exactly one block).
+      std::vector<unsigned int> max_age(fields.size(), 0);
+      for (const auto &b : blocks)
+        for (const auto *deps : {&b.dependencies_rhs, &b.dependencies_lhs})
+          for (const auto &[f, dep] : *deps)
+            max_age[f] = std::max(max_age[f], dep.oldest_age());
+      storage.resize(fields.size());
+      for (std::size_t f = 0; f < fields.size(); ++f)
+        {
+          if (max_age[f] > PPF_MAX_AGE)
+            throw Error(status_unsupported, "dependencies older than old_4 are not supported");
+          storage[f].resize(1 + max_age[f]);
+          for (auto &buf : storage[f])
+            {
+              buf.resize(fine.n_nodes);
+              fill(buf.get(), 0.0, fine.n_nodes);
+            }
+        }
+      sync();
+    }
+
+    /// Evaluates set_dirichlet at every Dirichlet node of each field; lower boundary ids
+    /// win at shared edges, as deal.II keeps the first constraint of a DoF.
+    void
+    compute_dirichlet_values(const GridDesc &g, std::vector<DeviceBuffer<double>> &out)
+    {
+      out.clear();
+      out.resize(fields.size());
+      for (std::size_t f = 0; f < fields.size(); ++f)
+        {
+          const unsigned int mask = dirichlet_mask(int(f));
+          if (!mask)
+            continue;
+          std::vector<double> host(g.n_nodes, 0.0);
+          for (int64_t node = 0; node < g.n_nodes; ++node)
+            {
+              int i[3];
+              i[0] = int(node % g.nodes[0]);
+              i[1] = int((node / g.nodes[0]) % g.nodes[1]);
+              i[2] = dim == 3 ? int(node / (int64_t(g.nodes[0]) * g.nodes[1])) : 0;
+              for (unsigned int bid = 0; bid < 2u * dim; ++bid)
+                if (on_masked_face(g, int(bid / 2), i[bid / 2], mask & (1u << bid)))
+                  {
+                    dealii::Point<dim> p;
+                    for (int d = 0; d < dim; ++d)
+                      p[d] = g.lower[d] + i[d] * g.h[d] / degree;
+                    double value = 0.0, vec = 0.0;
+                    pde_d->set_dirichlet(unsigned(f), bid, 0u, p, timer, value, vec);
+                    host[node] = value;
+                    break;
+                  }
+            }
+          out[f].resize(g.n_nodes);
+          PPF_HIP_CHECK(hipMemcpyAsync(out[f].get(), host.data(), sizeof(double) * g.n_nodes,
+                                       hipMemcpyHostToDevice, stream));
+        }
+      sync();
+    }
+
+    double *
+    field_ptr(int f, int age)
+    {
+      if (f < 0 || f >= int(fields.size()) || age < 0 || age >= int(storage[f].size()))
+        throw Error(status_invalid, "field/age out of range");
+      return storage[f][age].get();
+    }
+
+    /// Copies a field from host memory given over deal.II's full node set (periodic
+    /// images included, lexicographic, x fastest); image entries are ignored.
+    void
+    set_field(int f, int age, const double *host, int64_t n)
+    {
+      const GridDesc &g = fine;
+      if (n != full_size(g))
+        throw Error(status_invalid, "set_field: expected " + std::to_string(full_size(g)) +
+                                      " values");
+      std::vector<double> uniq(g.n_nodes);
+      const int64_t       f0 = int64_t(degree) * g.cells[0] + 1;
+      const int64_t       f1 = dim > 1 ? int64_t(degree) * g.cells[1] + 1 : 1;
+      for (int64_t node = 0; node < g.n_nodes; ++node)
+        {
+          const int64_t i0 = node % g.nodes[0];
+          const int64_t i1 = (node / g.nodes[0]) % g.nodes[1];
+          const int64_t i2 = dim == 3 ? node / (int64_t(g.nodes[0]) * g.nodes[1]) : 0;
+          uniq[node]       = host[i0 + f0 * (i1 + f1 * i2)];
+        }
+      PPF_HIP_CHECK(hipMemcpyAsync(field_ptr(f, age), uniq.data(), sizeof(double) * g.n_nodes,
+                                   hipMemcpyHostToDevice, stream));
+      sync();
+    }
+
+    void
+    get_field(int f, int age, double *host, int64_t n)
+    {
+      const GridDesc &g = fine;
+      if (n != full_size(g))
+        throw Error(status_invalid, "get_field: expected " + std::to_string(full_size(g)) +
+                                      " values");
+      std::vector<double> uniq(g.n_nodes);
+      PPF_HIP_CHECK(hipMemcpyAsync(uniq.data(), field_ptr(f, age), sizeof(double) * g.n_nodes,
+                                   hipMemcpyDeviceToHost, stream));
+      sync();
+      const int64_t f0 = int64_t(degree) * g.cells[0] + 1;
+      const int64_t f1 = dim > 1 ? int64_t(degree) * g.cells[1] + 1 : 1;
+      const int64_t f2 = dim > 2 ? int64_t(degree) * g.cells[2] + 1 : 1;
+      for (int64_t j2 = 0; j2 < f2; ++j2)
+        for (int64_t j1 = 0; j1 < f1; ++j1)
+          for (int64_t j0 = 0; j0 < f0; ++j0)
+            {
+              const int64_t i0 = j0 % g.nodes[0], i1 = j1 % g.nodes[1], i2 = j2 % g.nodes[2];
+              host[j0 + f0 * (j1 + f1 * j2)] = uniq[node_index(g, int(i0), int(i1), int(i2))];
+            }
+    }
+
+    // ---------------------------------------------------------------- utilities
+    void
+    sync()
+    {
+      PPF_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    template <typename T>
+    void
+    fill(T *x, T v, int64_t n)
+    {
+      if (n <= 0)
+        return;
+      hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, stream, x, v, n);
+    }
+
+    template <typename T>
+    void
+    copy(T *dst, const T *src, int64_t n)
+    {
+      PPF_HIP_CHECK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToDevice, stream));
+    }
+
+    template <typename T, typename U>
+    void
+    convert(T *dst, const U *src, int64_t n)
+    {
+      hipLaunchKernelGGL((convert_kernel<T, U>), dim3(grid_for(n, 256)), dim3(256), 0, stream,
+                         dst, src, n);
+    }
+
+    template <typename T>
+    void
+    axpby(T *y, T a, const T *x, T b, int64_t n)
+    {
+      hipLaunchKernelGGL(axpby_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, stream, y, a,
+                         x, b, n);
+    }
+
+    /// Deterministic dot product; the result is copied to the host.
+    template <typename T>
+    double
+    dot(const T *x, const T *y, int64_t n)
+    {
+      hipLaunchKernelGGL(dot_partial_kernel<T>, dim3(reduce_blocks), dim3(reduce_threads), 0,
+                         stream, reduce_buf.get(), x, y, n);
+      return finish_reduction(reduce_blocks);
+    }
+
+    double
+    finish_reduction(int count, int offset = 0)
+    {
+      hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(reduce_threads), 0, stream,
+                         scalar_buf.get(), reduce_buf.get() + offset, count);
+      double r = 0.0;
+      PPF_HIP_CHECK(hipMemcpyAsync(&r, scalar_buf.get(), sizeof(double), hipMemcpyDeviceToHost,
+                                   stream));
+      sync();
+      return r;
+    }
+
+    // ------------------------------------------------------------ operator plans
+    struct OpKey
+    {
+      int  block;
+      bool lhs;
+      int  precision; // 0 double, 1 float
+      bool
+      operator<(const OpKey &o) const
+      {
+        return std::tie(block, lhs, precision) < std::tie(o.block, o.lhs, o.precision);
+      }
+    };
+
+    struct PlanEntry
+    {
+      SplitPlan plan;
+      double    time = 0.0;
+      double    dt   = 0.0;
+    };
+
+    template <typename number>
+    const SplitPlan &
+    split_plan(int b, bool lhs)
+    {
+      const OpKey key {b, lhs, std::is_same_v<number, float> ? 1 : 0};
+      auto        it = plans.find(key);
+      if (it != plans.end() && !(it->second.plan.time_dependent &&
+                                 it->second.time != timer.get_time()) &&
+          it->second.dt == timer.get_timestep())
+        return it->second.plan;
+      const auto &pde = *pde_of<number>();
+      HookProbe<PDE<number>, dim, degree, number> probe(pde, blocks[b], lhs);
+      PlanEntry entry;
+      entry.plan = probe.analyse(timer.get_time(), timer.get_timestep());
+      entry.time = timer.get_time();
+      entry.dt   = timer.get_timestep();
+      return plans[key] = entry, plans[key].plan;
+    }
+
+    template <typename number>
+    const PDE<number> *
+    pde_of() const
+    {
+      if constexpr (std::is_same_v<number, float>)
+        return pde_f.get();
+      else
+        return pde_d.get();
+    }
+
+    /// Level-local view of the non-operand fields an operator reads.
+    template <typename number>
+    struct LevelFields
+    {
+      const GridDesc *grid = nullptr;
+      /// ptr[f][slot] for current/old slots; nullptr where not provided.
+      const number *ptr[PPF_MAX_FIELDS][PPF_N_SLOTS] = {};
+    };
+
+    LevelFields<double>
+    fine_fields()
+    {
+      LevelFields<double> lf;
+      lf.grid = &fine;
+      for (std::size_t f = 0; f < storage.size(); ++f)
+        for (std::size_t a = 0; a < storage[f].size(); ++a)
+          lf.ptr[f][a + 1] = storage[f][a].get();
+      return lf;
+    }
+
+    /// Applies the rhs or lhs operator of block `b`:
+    ///   dst = scaling * (assembled hook terms) with the Dirichlet rule `dwrite`.
+    /// `src[k]`/`dst[k]` belong to the k-th field of the block (ascending field index).
+    template <typename number>
+    void
+    apply_operator(int b, bool lhs, const LevelFields<number> &lf,
+                   const std::vector<const number *> &src, const std::vector<number *> &dst,
+                   int scaling, int dwrite, bool read_plain,
+                   const std::vector<DeviceBuffer<number>> *dvalues = nullptr)
+    {
+      const prismspf::SolveBlock &blk = blocks[b];
+      const GridDesc             &g   = *lf.grid;
+      DepTable<number>            deps {};
+      OutTable<number>            out {};
+      std::vector<int>            bf(blk.field_indices.begin(), blk.field_indices.end());
+      if (dst.size() != bf.size() || (!src.empty() && src.size() != bf.size()))
+        throw Error(status_invalid, "operator: block vector size mismatch");
+      for (std::size_t f = 0; f < fields.size(); ++f)
+        deps.dirichlet_mask[f] = dirichlet_mask(int(f));
+      constexpr int src_slot = prismspf::dependency_slot(prismspf::SRC);
+      for (const auto &in : declared_inputs(lhs ? blk.dependencies_lhs : blk.dependencies_rhs))
+        {
+          unsigned int  fl  = in.flags;
+          const number *ptr = nullptr;
+          if (in.slot == src_slot)
+            {
+              const auto pos = std::find(bf.begin(), bf.end(), in.field);
+              if (pos == bf.end())
+                throw Error(status_invalid, "src dependency on a field outside the block");
+              if (src.empty())
+                throw Error(status_invalid, "operator needs a src vector");
+              ptr = src[pos - bf.begin()];
+              if (!read_plain)
+                fl |= slot_constrained;
+            }
+          else
+            ptr = lf.ptr[in.field][in.slot];
+          if (!ptr)
+            throw Error(status_internal, "dependency storage not available on this level");
+          deps.ptr[in.field][in.slot]   = ptr;
+          deps.flags[in.field][in.slot] = fl;
+        }
+      for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+        out.scratch_slot[f] = -1;
+      for (std::size_t k = 0; k < bf.size(); ++k)
+        {
+          const int f            = bf[k];
+          out.dst[f]             = dst[k];
+          out.dirichlet_mask[f]  = deps.dirichlet_mask[f];
+          out.scratch_slot[f]    = int(k);
+          out.out_mask          |= 1u << f;
+          if (dwrite == dirichlet_value && out.dirichlet_mask[f])
+            {
+              if (!dvalues || !(*dvalues)[f].get())
+                throw Error(status_internal, "missing Dirichlet values");
+              out.dirichlet_values[f] = (*dvalues)[f].get();
+            }
+        }
+      out.scaling         = scaling;
+      out.dirichlet_write = dwrite;
+
+      const SplitPlan &plan = split_plan<number>(b, lhs);
+      const bool use_split  = path == OperatorPath::Split ||
+                             (path == OperatorPath::Auto && plan.eligible);
+      if (use_split && !plan.eligible)
+        throw Error(status_unsupported, "split path requested but the hook does not "
+                                        "qualify: " + plan.reason);
+      const auto        &pde = *pde_of<number>();
+      const unsigned int nb  = grid_for(g.n_nodes, block_size);
+      dispatch_block_id(typename App::BlockIds {}, blk.id, [&](auto bid) {
+        constexpr int id = decltype(bid)::value;
+        if (use_split)
+          {
+            if (lhs)
+              hipLaunchKernelGGL((split_operator_kernel<PDE<number>, dim, degree, number, true, id>),
+                                 dim3(nb), dim3(block_size), 0, stream, pde, g, deps, out,
+                                 plan.terms, timer);
+            else
+              hipLaunchKernelGGL((split_operator_kernel<PDE<number>, dim, degree, number, false, id>),
+                                 dim3(nb), dim3(block_size), 0, stream, pde, g, deps, out,
+                                 plan.terms, timer);
+          }
+        else
+          {
+            number *scratch = ensure_scratch<number>(int64_t(bf.size()) *
+                                                     dofs_per_cell<dim, degree> * g.n_cells);
+            const unsigned int nc = grid_for(g.n_cells, block_size);
+            if (lhs)
+              hipLaunchKernelGGL(
+                (cell_operator_kernel<PDE<number>, dim, degree, number, true, id, false, App::n_fields>),
+                dim3(nc), dim3(block_size), 0, stream, pde, g, deps, out, scratch, timer, -1);
+            else
+              hipLaunchKernelGGL(
+                (cell_operator_kernel<PDE<number>, dim, degree, number, false, id, false, App::n_fields>),
+                dim3(nc), dim3(block_size), 0, stream, pde, g, deps, out, scratch, timer, -1);
+            hipLaunchKernelGGL((node_gather_kernel<dim, degree, number>), dim3(nb),
+                               dim3(block_size), 0, stream, g, out, scratch);
+          }
+      });
+      PPF_HIP_CHECK(hipGetLastError());
+    }
+
+    /// Operator diagonal of the lhs of block `b` for block field k, scaled like the
+    /// operator; zero entries (Dirichlet rows) become one and the inverse is returned
+    /// (reference mf_operator.cc:82-255).
+    template <typename number>
+    void
+    inverse_diagonal(int b, const LevelFields<number> &lf, int scaling, number *inv_diag)
+    {
+      const prismspf::SolveBlock &blk = blocks[b];
+      const GridDesc             &g   = *lf.grid;
+      std::vector<int>            bf(blk.field_indices.begin(), blk.field_indices.end());
+      const SplitPlan            &plan = split_plan<number>(b, true);
+      const bool                  use_split =
+        path != OperatorPath::Generic && plan.eligible && plan.alpha_constant;
+      const unsigned int nb = grid_for(g.n_nodes, block_size);
+      for (std::size_t k = 0; k < bf.size(); ++k)
+        {
+          const int f   = bf[k];
+          number   *out = inv_diag + k * g.n_nodes;
+          if (use_split)
+            hipLaunchKernelGGL((split_diagonal_kernel<dim, degree, number>), dim3(nb),
+                               dim3(block_size), 0, stream, g, out, plan.alpha[f],
+                               plan.beta_src[f], dirichlet_mask(f), scaling);
+          else
+            generic_diagonal<number>(b, lf, int(k), scaling, out);
+        }
+      for (std::size_t k = 0; k < bf.size(); ++k)
+        {
+          number *out = inv_diag + k * g.n_nodes;
+          hipLaunchKernelGGL(invert_diagonal_kernel<number>, dim3(nb), dim3(block_size), 0,
+                             stream, out, out, g.n_nodes);
+        }
+      PPF_HIP_CHECK(hipGetLastError());
+    }
+
+    template <typename number>
+    void
+    generic_diagonal(int b, const LevelFields<number> &lf, int k, int scaling, number *out_ptr)
+    {
+      const prismspf::SolveBlock &blk = blocks[b];
+      const GridDesc             &g   = *lf.grid;
+      std::vector<int>            bf(blk.field_indices.begin(), blk.field_indices.end());
+      DepTable<number>            deps {};
+      OutTable<number>            out {};
+      constexpr int               src_slot = prismspf::dependency_slot(prismspf::SRC);
+      for (std::size_t f = 0; f < fields.size(); ++f)
+        deps.dirichlet_mask[f] = dirichlet_mask(int(f));
+      for (const auto &in : declared_inputs(blk.dependencies_lhs))
+        {
+          if (in.slot == src_slot)
+            {
+              // Values come from the unit probe inside the kernel.
+              deps.ptr[in.field][in.slot]   = nullptr;
+              deps.flags[in.field][in.slot] = 0;
+              continue;
+            }
+          deps.ptr[in.field][in.slot]   = lf.ptr[in.field][in.slot];
+          deps.flags[in.field][in.slot] = in.flags;
+        }
+      for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+        out.scratch_slot[f] = -1;
+      const int f           = bf[k];
+      out.scratch_slot[f]   = 0;
+      out.dst[f]            = out_ptr;
+      out.dirichlet_mask[f] = deps.dirichlet_mask[f];
+      out.out_mask          = 1u << f;
+      out.scaling           = scaling;
+      out.dirichlet_write   = dirichlet_zero;
+      number *scratch       = ensure_scratch<number>(int64_t(dofs_per_cell<dim, degree>) * g.n_cells);
+      const auto &pde       = *pde_of<number>();
+      dispatch_block_id(typename App::BlockIds {}, blk.id, [&](auto bid) {
+        constexpr int id = decltype(bid)::value;
+        hipLaunchKernelGGL((cell_operator_kernel<PDE<number>, dim, degree, number, true, id, true, App::n_fields>),
+                           dim3(grid_for(g.n_cells, block_size)), dim3(block_size), 0, stream,
+                           pde, g, deps, out, scratch, timer, f);
+      });
+      hipLaunchKernelGGL((node_gather_kernel<dim, degree, number>),
+                         dim3(grid_for(g.n_nodes, block_size)), dim3(block_size), 0, stream, g,
+                         out, scratch);
+    }
+
+    template <typename number>
+    number *
+    ensure_scratch(int64_t n)
+    {
+      if constexpr (std::is_same_v<number, float>)
+        {
+          if (scratch_f.size() < n)
+            scratch_f.resize(n);
+          return scratch_f.get();
+        }
+      else
+        {
+          if (scratch_d.size() < n)
+            scratch_d.resize(n);
+          return scratch_d.get();
+        }
+    }
+
+    // ----------------------------------------------------------- explicit solve
+    /// ExplicitSolver::solve_impl (reference explicit_solver.h:63-77): the block's fields
+    /// become invm * (assembled rhs), with Dirichlet nodes set to their values.
+    void
+    solve_explicit(int b)
+    {
+      std::vector<double *> dst;
+      for (auto f : blocks[b].field_indices)
+        dst.push_back(field_ptr(int(f), 0));
+      apply_operator<double>(b, false, fine_fields(), {}, dst, scale_invm, dirichlet_value,
+                             false, &dirichlet_values);
+    }
+
+    // ------------------------------------------------------------- linear solve
+    struct Level
+    {
+      GridDesc                            grid;
+      int                                 precision = 0;
+      DeviceBuffer<double>                vd[6];
+      DeviceBuffer<float>                 vf[6];
+      DeviceBuffer<double>                inv_diag_d;
+      DeviceBuffer<float>                 inv_diag_f;
+      double                              lambda_max = 0.0, lambda_min = 0.0;
+      std::vector<DeviceBuffer<double>>   field_d[PPF_MAX_FIELDS];
+      std::vector<DeviceBuffer<float>>    field_f[PPF_MAX_FIELDS];
+    };
+
+    struct LinearState
+    {
+      bool                 initialized = false;
+      int64_t              n           = 0; // block vector length
+      DeviceBuffer<double> x_homog, inhom, rhs, g, h, d, ad, tmp;
+      DeviceBuffer<double> inv_diag; // Chebyshev preconditioner on the fine level
+      double               cheb_lmax = 0.0, cheb_lmin = 0.0;
+      DeviceBuffer<double> cheb_d, cheb_ax;
+      std::vector<Level>   levels; // coarsest first
+      SolveStats           stats;
+    };
+
+    const prismspf::LinearSolverParameters &
+    lin_params(int b) const
+    {
+      return blocks[b].linear_solver_parameters;
+    }
+
+    int
+    linear_scaling(int b) const
+    {
+      return lin_params(b).tolerance_type != prismspf::AbsoluteResidual ? scale_invm_sqrt
+                                                                        : scale_none;
+    }
+
+    /// normalization_value (reference linear_solver.h:414-427).
+    double
+    normalization(int b) const
+    {
+      const auto type = lin_params(b).tolerance_type;
+      double     v    = 1.0;
+      if (type == prismspf::RMSEPerField || type == prismspf::RMSETotal)
+        {
+          double vol = 1.0;
+          for (int d = 0; d < dim; ++d)
+            vol *= fine.h[d] * fine.cells[d];
+          v *= std::sqrt(vol);
+        }
+      if (type == prismspf::RMSEPerField || type == prismspf::IntegratedPerField)
+        v *= std::sqrt(double(blocks[b].field_indices.size()));
+      return v;
+    }
+
+    std::vector<double *>
+    split_block(double *base, int b, const GridDesc &g) const
+    {
+      std::vector<double *> v;
+      for (std::size_t k = 0; k < blocks[b].field_indices.size(); ++k)
+        v.push_back(base + k * g.n_nodes);
+      return v;
+    }
+
+    template <typename T>
+    std::vector<T *>
+    split_level(T *base, int b, const GridDesc &g) const
+    {
+      std::vector<T *> v;
+      for (std::size_t k = 0; k < blocks[b].field_indices.size(); ++k)
+        v.push_back(base + k * g.n_nodes);
+      return v;
+    }
+
+    LinearState &
+    linear_state(int b)
+    {
+      LinearState &ls = linear_states[b];
+      if (ls.initialized)
+        return ls;
+      const int64_t n = int64_t(blocks[b].field_indices.size()) * fine.n_nodes;
+      ls.n            = n;
+      for (auto *buf : {&ls.x_homog, &ls.inhom, &ls.rhs, &ls.g, &ls.h, &ls.d, &ls.ad, &ls.tmp})
+        {
+          buf->resize(n);
+          fill(buf->get(), 0.0, n);
+        }
+      ls.initialized = true;
+      return ls;
+    }
+
+    /// Full-block vmult with the fine FP64 lhs operator (homogeneous constraints).
+    void
+    vmult_fine(int b, double *dst, const double *src)
+    {
+      auto s = split_block(const_cast<double *>(src), b, fine);
+      apply_operator<double>(b, true, fine_fields(), {s.begin(), s.end()},
+                             split_block(dst, b, fine), linear_scaling(b), dirichlet_zero,
+                             false);
+    }
+
+    /// LinearSolver::solve_impl (reference linear_solver.h:305-345).
+    SolveStats
+    solve_linear(int b)
+    {
+      LinearState &ls = linear_state(b);
+      const auto  &bf = blocks[b].field_indices;
+      const int    sc = linear_scaling(b);
+
+      // rhs = scaling * rhs operator
+      apply_operator<double>(b, false, fine_fields(), {}, split_block(ls.rhs.get(), b, fine), sc,
+                             dirichlet_zero, false);
+      // inhomogeneous values: g at Dirichlet nodes, 0 elsewhere
+      {
+        int k = 0;
+        for (auto f : bf)
+          {
+            double *dstp = ls.inhom.get() + k * fine.n_nodes;
+            fill(dstp, 0.0, fine.n_nodes);
+            if (dirichlet_mask(int(f)))
+              hipLaunchKernelGGL((dirichlet_compose_kernel<dim, double>),
+                                 dim3(grid_for(fine.n_nodes, 256)), dim3(256), 0, stream, dstp,
+                                 fine, dirichlet_values[f].get(), dirichlet_mask(int(f)), 0.0, 1);
+            ++k;
+          }
+      }
+      // rhs -= A_plain(inhom)
+      {
+        auto s = split_block(ls.inhom.get(), b, fine);
+        apply_operator<double>(b, true, fine_fields(), {s.begin(), s.end()},
+                               split_block(ls.tmp.get(), b, fine), sc, dirichlet_zero, true);
+        axpby(ls.rhs.get(), -1.0, ls.tmp.get(), 1.0, ls.n);
+      }
+
+      prepare_preconditioner(b);
+      SolveStats st = cg(b, ls.x_homog.get(), ls.rhs.get(),
+                         lin_params(b).tolerance * normalization(b), lin_params(b).max_iterations);
+
+      // solution = x + inhomogeneous values
+      int k = 0;
+      for (auto f : bf)
+        {
+          double *sol = field_ptr(int(f), 0);
+          copy(sol, ls.x_homog.get() + k * fine.n_nodes, fine.n_nodes);
+          axpby(sol, 1.0, ls.inhom.get() + k * fine.n_nodes, 1.0, fine.n_nodes);
+          ++k;
+        }
+      sync();
+      ls.stats = st;
+      return st;
+    }
+
+    /// Preconditioned CG with deal.II SolverCG's recurrences and stopping rule
+    /// (absolute residual tolerance; step 0 checks the initial residual).
+    SolveStats
+    cg(int b, double *x, const double *rhs, double tol, unsigned int max_steps)
+    {
+      LinearState &ls = linear_state(b);
+      const int64_t n = ls.n;
+      SolveStats    st;
+      st.tolerance = tol;
+      // g = A x - b
+      vmult_fine(b, ls.g.get(), x);
+      axpby(ls.g.get(), -1.0, rhs, 1.0, n);
+      double res          = std::sqrt(dot(ls.g.get(), ls.g.get(), n));
+      st.initial_residual = res;
+      unsigned int step   = 0;
+      if (!(res <= tol))
+        {
+          precondition(b, ls.h.get(), ls.g.get());
+          axpby(ls.d.get(), -1.0, ls.h.get(), 0.0, n);
+          double gh = dot(ls.g.get(), ls.h.get(), n);
+          while (true)
+            {
+              ++step;
+              vmult_fine(b, ls.ad.get(), ls.d.get());
+              const double dad   = dot(ls.d.get(), ls.ad.get(), n);
+              const double alpha = gh / dad;
+              hipLaunchKernelGGL(cg_update_kernel<double>, dim3(reduce_blocks),
+                                 dim3(reduce_threads), 0, stream, reduce_buf.get(), x,
+                                 ls.g.get(), ls.d.get(), ls.ad.get(), alpha, n);
+              res = std::sqrt(finish_reduction(reduce_blocks));
+              if (!std::isfinite(res))
+                throw Error(status_nan, "linear solver produced a non-finite residual");
+              if (res <= tol || step >= max_steps)
+                break;
+              precondition(b, ls.h.get(), ls.g.get());
+              const double gh_new = dot(ls.g.get(), ls.h.get(), n);
+              const double beta   = gh_new / gh;
+              gh                  = gh_new;
+              hipLaunchKernelGGL(cg_direction_kernel<double>, dim3(grid_for(n, 256)), dim3(256),
+                                 0, stream, ls.d.get(), ls.h.get(), beta, n);
+            }
+        }
+      st.iterations     = int(step);
+      st.final_residual = res;
+      st.converged      = res <= tol ? 1 : 0;
+      return st;
+    }
+
+    // ------------------------------------------------------------ preconditioners
+    void
+    prepare_preconditioner(int b)
+    {
+      LinearState &ls   = linear_state(b);
+      const auto   kind = lin_params(b).preconditioner;
+      if (kind == prismspf::Chebyshev)
+        {
+          // The reference re-evaluates the diagonal every solve (linear_solver.h:362-378).
+          ls.inv_diag.resize(ls.n);
+          ls.cheb_d.resize(ls.n);
+          ls.cheb_ax.resize(ls.n);
+          inverse_diagonal<double>(b, fine_fields(), linear_scaling(b), ls.inv_diag.get());
+          estimate_eigenvalues<double>(
+            [&](double *dst, const double *src) { vmult_fine(b, dst, src); }, ls.inv_diag.get(),
+            ls.n, lin_params(b).chebyshev_parameters, ls.cheb_lmax, ls.cheb_lmin, ls.tmp.get(),
+            ls.cheb_d.get(), ls.cheb_ax.get(), ls.h.get());
+        }
+      else if (kind == prismspf::GMG)
+        setup_multigrid(b);
+    }
+
+    void
+    precondition(int b, double *h, const double *g)
+    {
+      LinearState &ls   = linear_state(b);
+      const auto   kind = lin_params(b).preconditioner;
+      if (kind == prismspf::None)
+        copy(h, g, ls.n);
+      else if (kind == prismspf::Chebyshev)
+        chebyshev<double>(
+          [&](double *dst, const double *src) { vmult_fine(b, dst, src); }, ls.inv_diag.get(),
+          ls.n, lin_params(b).chebyshev_parameters.degree, ls.cheb_lmax, ls.cheb_lmin, h, g,
+          ls.cheb_d.get(), ls.cheb_ax.get(), true);
+      else
+        multigrid_vmult(b, h, g);
+    }
+
+    /// Lanczos estimate of the largest eigenvalue of P^{-1} A from `its` CG steps (the
+    /// approach of deal.II's PreconditionChebyshev); lmax gets a 1.2 safety factor and
+    /// lmin = lmax / smoothing_range.  The start vector is a fixed pseudo-random vector
+    /// with zero Dirichlet rows (parity with deal.II's start vector is not claimed).
+    template <typename T, class Vmult>
+    void
+    estimate_eigenvalues(Vmult &&A, const T *inv_diag, int64_t n,
+                         const prismspf::ChebyshevParameters &cp, double &lmax, double &lmin,
+                         T *r, T *p, T *ap, T *z)
+    {
+      std::vector<T> host(n);
+      uint64_t       s = 0x243f6a8885a308d3ull;
+      for (int64_t i = 0; i < n; ++i)
+        {
+          s ^= s << 13;
+          s ^= s >> 7;
+          s ^= s << 17;
+          host[i] = T(double(s >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+        }
+      PPF_HIP_CHECK(hipMemcpyAsync(z, host.data(), sizeof(T) * n, hipMemcpyHostToDevice, stream));
+      // CG for A e = 0 from the pseudo-random guess: the initial residual is A x0, which
+      // also has zero constrained rows.  z = P^{-1} r.
+      A(r, z);
+      hipLaunchKernelGGL(diag_mult_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, stream, z,
+                         inv_diag, r, n);
+      copy(p, z, n);
+      double              rz = dot(r, z, n);
+      std::vector<double> alphas, betas;
+      for (unsigned int it = 0; it < cp.eig_cg_n_iterations && rz > 0.0; ++it)
+        {
+          A(ap, p);
+          const double pap = dot(p, ap, n);
+          if (!(pap > 0.0))
+            break;
+          const double alpha = rz / pap;
+          alphas.push_back(alpha);
+          axpby(r, T(-alpha), ap, T(1), n);
+          hipLaunchKernelGGL(diag_mult_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, stream, z,
+                             inv_diag, r, n);
+          const double rz_new = dot(r, z, n);
+          const double beta   = rz_new / rz;
+          betas.push_back(beta);
+          rz = rz_new;
+          axpby(p, T(1), z, T(beta), n);
+        }
+      // Lanczos tridiagonal from the CG coefficients.
+      const int           m = int(alphas.size());
+      std::vector<double> diag(m), off(std::max(0, m - 1));
+      for (int i = 0; i < m; ++i)
+        {
+          diag[i] = 1.0 / alphas[i] + (i > 0 ? betas[i - 1] / alphas[i - 1] : 0.0);
+          if (i + 1 < m)
+            off[i] = std::sqrt(std::max(0.0, betas[i])) / alphas[i];
+        }
+      const double est = m > 0 ? tridiagonal_max_eigenvalue(diag, off) : 1.0;
+      lmax             = 1.2 * est;
+      lmin             = lmax / cp.smoothing_range;
+    }
+
+    /// Chebyshev iteration of the given degree for A x = rhs preconditioned by the
+    /// inverse diagonal.  With `zero_start` x is overwritten (preconditioner use,
+    /// deal.II vmult); otherwise x is the starting guess (smoother use, deal.II step).
+    template <typename T, class Vmult>
+    void
+    chebyshev(Vmult &&A, const T *inv_diag, int64_t n, unsigned int deg, double lmax,
+              double lmin, T *x, const T *rhs, T *d, T *ax, bool zero_start)
+    {
+      const double theta = 0.5 * (lmax + lmin);
+      const double delta = 0.5 * (lmax - lmin);
+      const double sigma = theta / delta;
+      double       rho   = 1.0 / sigma;
+      if (zero_start)
+        fill(x, T(0), n);
+      for (unsigned int k = 0; k < deg; ++k)
+        {
+          if (k == 0 && zero_start)
+            fill(ax, T(0), n);
+          else
+            A(ax, x);
+          double c1 = 0.0, c2;
+          if (k == 0)
+            c2 = 1.0 / theta;
+          else
+            {
+              const double rho_new = 1.0 / (2.0 * sigma - rho);
+              c1                   = rho_new * rho;
+              c2                   = 2.0 * rho_new / delta;
+              rho                  = rho_new;
+            }
+          hipLaunchKernelGGL(chebyshev_step_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0,
+                             stream, x, d, rhs, ax, inv_diag, T(c1), T(c2), k == 0 ? 1 : 0, n);
+        }
+    }
+
+    // ----------------------------------------------------------------- multigrid
+    template <typename T>
+    T *
+    lvec(Level &l, int i)
+    {
+      if constexpr (std::is_same_v<T, float>)
+        return l.vf[i].get();
+      else
+        return l.vd[i].get();
+    }
+
+    template <typename T>
+    T *
+    linv(Level &l)
+    {
+      if constexpr (std::is_same_v<T, float>)
+        return l.inv_diag_f.get();
+      else
+        return l.inv_diag_d.get();
+    }
+
+    template <typename T>
+    LevelFields<T>
+    level_fields(Level &l)
+    {
+      LevelFields<T> lf;
+      lf.grid = &l.grid;
+      for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+        {
+          if constexpr (std::is_same_v<T, float>)
+            {
+              for (std::size_t a = 0; a < l.field_f[f].size(); ++a)
+                lf.ptr[f][a + 1] = l.field_f[f][a].get();
+            }
+          else
+            {
+              for (std::size_t a = 0; a < l.field_d[f].size(); ++a)
+                lf.ptr[f][a + 1] = l.field_d[f][a].get();
+            }
+        }
+      return lf;
+    }
+
+    template <typename T>
+    void
+    level_vmult(int b, Level &l, T *dst, const T *src)
+    {
+      auto s = split_level<T>(const_cast<T *>(src), b, l.grid);
+      apply_operator<T>(b, true, level_fields<T>(l), {s.begin(), s.end()},
+                        split_level<T>(dst, b, l.grid), level_scaling(b), dirichlet_zero, false);
+    }
+
+    int
+    level_scaling(int b) const
+    {
+      return linear_scaling(b);
+    }
+
+    /// MGContext::init (reference linear_solver.h:86-165): level operators, level
+    /// diagonals, Chebyshev smoothers, transfers.  Levels span
+    /// [global_refinement - mg_depth + 1, global_refinement].
+    void
+    setup_multigrid(int b)
+    {
+      LinearState &ls    = linear_state(b);
+      const auto  &lp    = lin_params(b);
+      const int    top   = int(inputs.spatial_discretization.global_refinement);
+      const int    depth = int(lp.mg_depth);
+      if (depth < 1 || depth > top + 1)
+        throw Error(status_invalid, "mg depth must be in [1, global refinement + 1]");
+      const int  prec    = lp.level_precision == prismspf::LevelPrecision::Single ? 1 : 0;
+      const bool rebuild = ls.levels.size() != std::size_t(depth) ||
+                           (!ls.levels.empty() && ls.levels[0].precision != prec);
+      if (rebuild)
+        {
+          ls.levels.clear();
+          ls.levels.resize(depth);
+          for (int i = 0; i < depth; ++i)
+            {
+              Level &l    = ls.levels[i];
+              l.grid      = make_grid(unsigned(top - depth + 1 + i));
+              l.precision = prec;
+              const int64_t n = int64_t(blocks[b].field_indices.size()) * l.grid.n_nodes;
+              for (int v = 0; v < 6; ++v)
+                {
+                  if (prec)
+                    l.vf[v].resize(n);
+                  else
+                    l.vd[v].resize(n);
+                }
+              if (prec)
+                l.inv_diag_f.resize(n);
+              else
+                l.inv_diag_d.resize(n);
+            }
+        }
+      transfer_fields_to_levels(b, ls);
+      for (auto &l : ls.levels)
+        {
+          if (prec)
+            setup_level<float>(b, l);
+          else
+            setup_level<double>(b, l);
+        }
+      // FP64 coarse-grid CG storage lives on the coarsest level.
+      if (lp.coarse_solver == prismspf::CoarseSolver::CG && prec)
+        {
+          Level        &c = ls.levels[0];
+          const int64_t n = int64_t(blocks[b].field_indices.size()) * c.grid.n_nodes;
+          for (int v = 0; v < 6; ++v)
+            c.vd[v].resize(n);
+          c.inv_diag_d.resize(n);
+          inverse_diagonal<double>(b, level_fields<double>(c), level_scaling(b),
+                                   c.inv_diag_d.get());
+        }
+    }
+
+    /// Non-operand dependencies of the lhs are injected to every level
+    /// (reference group_solution_handler.h:356-417).
+    void
+    transfer_fields_to_levels(int b, LinearState &ls)
+    {
+      const auto &deps = blocks[b].dependencies_lhs;
+      for (auto &l : ls.levels)
+        for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+          {
+            l.field_d[f].clear();
+            l.field_f[f].clear();
+          }
+      for (const auto &in : declared_inputs(deps))
+        {
+          if (in.slot == prismspf::dependency_slot(prismspf::SRC))
+            continue;
+          const int age = in.slot - 1;
+          for (auto &l : ls.levels)
+            {
+              const bool want_f = l.precision == 1;
+              auto      &vd     = l.field_d[in.field];
+              auto      &vf     = l.field_f[in.field];
+              if (vd.size() <= std::size_t(age))
+                vd.resize(age + 1);
+              if (vf.size() <= std::size_t(age))
+                vf.resize(age + 1);
+              vd[age].resize(l.grid.n_nodes);
+              hipLaunchKernelGGL((inject_kernel<dim, double, double>),
+                                 dim3(grid_for(l.grid.n_nodes, 256)), dim3(256), 0, stream,
+                                 vd[age].get(), l.grid, field_ptr(in.field, age), fine);
+              if (want_f)
+                {
+                  vf[age].resize(l.grid.n_nodes);
+                  convert(vf[age].get(), vd[age].get(), l.grid.n_nodes);
+                }
+            }
+        }
+    }
+
+    template <typename T>
+    void
+    setup_level(int b, Level &l)
+    {
+      const auto   &lp = lin_params(b);
+      const int64_t n  = int64_t(blocks[b].field_indices.size()) * l.grid.n_nodes;
+      inverse_diagonal<T>(b, level_fields<T>(l), level_scaling(b), linv<T>(l));
+      estimate_eigenvalues<T>([&](T *dst, const T *src) { level_vmult<T>(b, l, dst, src); },
+                              linv<T>(l), n, lp.chebyshev_parameters, l.lambda_max, l.lambda_min,
+                              lvec<T>(l, 2), lvec<T>(l, 3), lvec<T>(l, 4), lvec<T>(l, 5));
+    }
+
+    /// PreconditionMG::vmult: one V-cycle on the residual g.
+    void
+    multigrid_vmult(int b, double *h, const double *g)
+    {
+      LinearState &ls  = linear_state(b);
+      Level       &top = ls.levels.back();
+      if (top.precision)
+        {
+          convert(lvec<float>(top, 1), g, ls.n);
+          v_cycle<float>(b, int(ls.levels.size()) - 1);
+          convert(h, lvec<float>(top, 0), ls.n);
+        }
+      else
+        {
+          copy(lvec<double>(top, 1), g, ls.n);
+          v_cycle<double>(b, int(ls.levels.size()) - 1);
+          copy(h, lvec<double>(top, 0), ls.n);
+        }
+    }
+
+    /// Multigrid::level_v_step.  Level vectors: 0 solution, 1 defect, 2 scratch t,
+    /// 3-4 smoother temporaries, 5 spare.
+    template <typename T>
+    void
+    v_cycle(int b, int li)
+    {
+      LinearState  &ls = linear_state(b);
+      Level        &l  = ls.levels[li];
+      const int64_t n  = int64_t(blocks[b].field_indices.size()) * l.grid.n_nodes;
+      const auto   &lp = lin_params(b);
+      auto A = [&](T *dst, const T *src) { level_vmult<T>(b, l, dst, src); };
+      if (li == 0)
+        {
+          coarse_solve<T>(b, l);
+          return;
+        }
+      const unsigned int deg = lp.chebyshev_parameters.degree;
+      chebyshev<T>(A, linv<T>(l), n, deg, l.lambda_max, l.lambda_min, lvec<T>(l, 0),
+                   lvec<T>(l, 1), lvec<T>(l, 3), lvec<T>(l, 4), true);
+      // t = defect - A u
+      A(lvec<T>(l, 2), lvec<T>(l, 0));
+      axpby(lvec<T>(l, 2), T(1), lvec<T>(l, 1), T(-1), n);
+      Level &c = ls.levels[li - 1];
+      for (std::size_t k = 0; k < blocks[b].field_indices.size(); ++k)
+        {
+          const int f = int(*std::next(blocks[b].field_indices.begin(), k));
+          hipLaunchKernelGGL((restrict_kernel<dim, degree, T>),
+                             dim3(grid_for(c.grid.n_nodes, 256)), dim3(256), 0, stream,
+                             lvec<T>(c, 1) + k * c.grid.n_nodes, c.grid,
+                             lvec<T>(l, 2) + k * l.grid.n_nodes, l.grid, dirichlet_mask(f));
+        }
+      v_cycle<T>(b, li - 1);
+      for (std::size_t k = 0; k < blocks[b].field_indices.size(); ++k)
+        {
+          const int f = int(*std::next(blocks[b].field_indices.begin(), k));
+          hipLaunchKernelGGL((prolongate_add_kernel<dim, degree, T>),
+                             dim3(grid_for(l.grid.n_nodes, 256)), dim3(256), 0, stream,
+                             lvec<T>(l, 0) + k * l.grid.n_nodes, l.grid,
+                             lvec<T>(c, 0) + k * c.grid.n_nodes, c.grid, dirichlet_mask(f));
+        }
+      chebyshev<T>(A, linv<T>(l), n, deg, l.lambda_max, l.lambda_min, lvec<T>(l, 0),
+                   lvec<T>(l, 1), lvec<T>(l, 3), lvec<T>(l, 4), false);
+    }
+
+    template <typename T>
+    void
+    coarse_solve(int b, Level &l)
+    {
+      const auto   &lp = lin_params(b);
+      const int64_t n  = int64_t(blocks[b].field_indices.size()) * l.grid.n_nodes;
+      if (lp.coarse_solver == prismspf::CoarseSolver::Smoother)
+        {
+          // MGCoarseGridApplySmoother (reference linear_solver.h:65,145).
+          chebyshev<T>([&](T *dst, const T *src) { level_vmult<T>(b, l, dst, src); },
+                       linv<T>(l), n, lp.chebyshev_parameters.degree, l.lambda_max,
+                       l.lambda_min, lvec<T>(l, 0), lvec<T>(l, 1), lvec<T>(l, 3), lvec<T>(l, 4),
+                       true);
+          return;
+        }
+      // FP64 CG with the diagonal preconditioner.
+      double *x = l.vd[0].get(), *r = l.vd[1].get(), *p = l.vd[2].get(), *ap = l.vd[3].get(),
+             *z = l.vd[4].get();
+      if constexpr (std::is_same_v<T, float>)
+        convert(r, l.vf[1].get(), n);
+      auto A = [&](double *dst, const double *src) { level_vmult<double>(b, l, dst, src); };
+      fill(x, 0.0, n);
+      const double r0 = std::sqrt(dot(r, r, n));
+      if (r0 > 0.0)
+        {
+          hipLaunchKernelGGL(diag_mult_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0,
+                             stream, z, l.inv_diag_d.get(), r, n);
+          copy(p, z, n);
+          double rz = dot(r, z, n);
+          for (unsigned int it = 0; it < lp.coarse_cg_max_iterations; ++it)
+            {
+              A(ap, p);
+              const double alpha = rz / dot(p, ap, n);
+              axpby(x, alpha, p, 1.0, n);
+              axpby(r, -alpha, ap, 1.0, n);
+              if (std::sqrt(dot(r, r, n)) <= lp.coarse_cg_tolerance * r0)
+                break;
+              hipLaunchKernelGGL(diag_mult_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0,
+                                 stream, z, l.inv_diag_d.get(), r, n);
+              const double rz_new = dot(r, z, n);
+              axpby(p, 1.0, z, rz_new / rz, n);
+              rz = rz_new;
+            }
+        }
+      if constexpr (std::is_same_v<T, float>)
+        convert(l.vf[0].get(), x, n);
+    }
+
+    // -------------------------------------------------------------- time loop
+    /// Initial conditions at the nodes (reference solver_base.h:179-229 interpolates the
+    /// user function at the support points).  Nodes are visited in lexicographic order;
+    /// periodic images take the master's value.
+    void
+    apply_initial_conditions(int b)
+    {
+      for (auto f : blocks[b].field_indices)
+        {
+          std::vector<double> host(fine.n_nodes);
+          for (int64_t node = 0; node < fine.n_nodes; ++node)
+            {
+              int i[3];
+              i[0] = int(node % fine.nodes[0]);
+              i[1] = int((node / fine.nodes[0]) % fine.nodes[1]);
+              i[2] = dim == 3 ? int(node / (int64_t(fine.nodes[0]) * fine.nodes[1])) : 0;
+              dealii::Point<dim> p;
+              for (int d = 0; d < dim; ++d)
+                p[d] = fine.lower[d] + i[d] * fine.h[d] / degree;
+              double value = 0.0, vec = 0.0;
+              pde_d->set_initial_condition(f, 0u, p, value, vec);
+              host[node] = value;
+            }
+          PPF_HIP_CHECK(hipMemcpyAsync(field_ptr(int(f), 0), host.data(),
+                                       sizeof(double) * fine.n_nodes, hipMemcpyHostToDevice,
+                                       stream));
+          sync();
+        }
+    }
+
+    void
+    copy_current_to_old(int b)
+    {
+      for (auto f : blocks[b].field_indices)
+        for (std::size_t a = 1; a < storage[f].size(); ++a)
+          copy(storage[f][a].get(), storage[f][0].get(), fine.n_nodes);
+    }
+
+    /// SolverBase::solve (reference solver_base.h:106-130).
+    void
+    solve_block(int b)
+    {
+      const auto &blk = blocks[b];
+      if (timer.get_increment() == 0 && blk.solve_timing == prismspf::Primary)
+        apply_initial_conditions(b);
+      else if (blk.solve_type == prismspf::Explicit)
+        solve_explicit(b);
+      else if (blk.solve_type == prismspf::Linear)
+        solve_linear(b);
+      else if (blk.solve_type == prismspf::Newton)
+        throw Error(status_unsupported, "Newton solve blocks are not supported yet");
+      if (timer.get_increment() == 0)
+        copy_current_to_old(b);
+    }
+
+    /// Old-state rotation by pointer swaps (reference group_solution_handler.cc:369-403).
+    void
+    rotate(int b)
+    {
+      for (auto f : blocks[b].field_indices)
+        for (int age = int(storage[f].size()) - 1; age >= 1; --age)
+          storage[f][age].swap(storage[f][age - 1]);
+    }
+
+    /// Problem::solve_increment (reference problem.cc:414-558).  Returns the exit status:
+    /// 0 normal, 2 NaN detected, 3 user stop.
+    int
+    solve_increment()
+    {
+      const unsigned int inc       = timer.get_increment();
+      const bool         do_output = inputs.output_parameters.should_output(inc);
+      for (std::size_t b = 0; b < blocks.size(); ++b)
+        {
+          const auto timing = blocks[b].solve_timing;
+          if ((timing == prismspf::PostProcess || timing == prismspf::NucleationRate) &&
+              !do_output)
+            continue;
+          solve_block(int(b));
+        }
+      int status = 0;
+      for (std::size_t f = 0; f < fields.size() && status == 0; ++f)
+        {
+          hipLaunchKernelGGL(finite_probe_kernel<double>, dim3(1), dim3(1), 0, stream,
+                             reinterpret_cast<int *>(scalar_buf.get() + 1), field_ptr(int(f), 0));
+          int flag = 0;
+          PPF_HIP_CHECK(hipMemcpyAsync(&flag, scalar_buf.get() + 1, sizeof(int),
+                                       hipMemcpyDeviceToHost, stream));
+          sync();
+          if (flag)
+            status = 2;
+        }
+      if (status == 0 && pde_d->get_user_stop())
+        status = 3;
+      if (do_output || status != 0)
+        log_norms();
+      for (std::size_t b = 0; b < blocks.size(); ++b)
+        rotate(int(b));
+      return status;
+    }
+
+    /// Problem::solve (reference problem.cc:340-412).
+    int
+    run(unsigned int n_increments)
+    {
+      int status = 0;
+      while (timer.get_increment() <= n_increments && status == 0)
+        {
+          status = solve_increment();
+          timer.increment();
+        }
+      sync();
+      return status;
+    }
+
+    /// l2-norm over deal.II's node set and the integral sum(u * jxw) of a field
+    /// (reference problem.cc:517-549).
+    std::pair<double, double>
+    norms(int f, int age = 0)
+    {
+      const GridDesc &g = fine;
+      hipLaunchKernelGGL((norms_partial_kernel<dim, degree, double>), dim3(reduce_blocks),
+                         dim3(reduce_threads), 0, stream, reduce_buf.get(), g,
+                         field_ptr(f, age));
+      const double s2 = finish_reduction(reduce_blocks, 0);
+      const double si = finish_reduction(reduce_blocks, reduce_blocks);
+      return {std::sqrt(s2), si};
+    }
+
+    void
+    log_norms()
+    {
+      std::ostringstream os;
+      os.precision(12);
+      os << "Iteration " << timer.get_increment() << " Time " << timer.get_time() << "\n";
+      for (std::size_t f = 0; f < fields.size(); ++f)
+        {
+          const auto [l2, integral] = norms(int(f));
+          os << "  Field index " << f << " name " << fields[f].name << " l2-norm " << l2
+             << " integrated value " << integral << "\n";
+        }
+      log += os.str();
+      if (std::getenv("PPF_VERBOSE"))
+        std::fputs(os.str().c_str(), stdout);
+    }
+
+    int
+    block_index(int id) const
+    {
+      for (std::size_t b = 0; b < blocks.size(); ++b)
+        if (blocks[b].id == id)
+          return int(b);
+      throw Error(status_invalid, "no solve block with id " + std::to_string(id));
+    }
+
+    // ------------------------------------------------------------------- state
+    prismspf::UserInputParameters<dim>    inputs;
+    prismspf::PhaseFieldTools<dim>        tools;
+    std::vector<prismspf::FieldAttributes> fields;
+    std::vector<prismspf::SolveBlock>      blocks;
+    std::unique_ptr<PDE<double>>           pde_d;
+    std::unique_ptr<PDE<float>>            pde_f;
+    prismspf::SimulationTimer              timer;
+    GridDesc                               fine {};
+    hipStream_t                            stream = nullptr;
+    OperatorPath                           path   = OperatorPath::Auto;
+
+    std::vector<std::vector<DeviceBuffer<double>>> storage;
+    std::vector<DeviceBuffer<double>>              dirichlet_values;
+    std::map<int, LinearState>                     linear_states;
+    std::map<OpKey, PlanEntry>                     plans;
+    DeviceBuffer<double>                           scratch_d;
+    DeviceBuffer<float>                            scratch_f;
+    DeviceBuffer<double>                           reduce_buf;
+    DeviceBuffer<double>                           scalar_buf;
+    std::string                                    log;
+  };
+} // namespace ppf

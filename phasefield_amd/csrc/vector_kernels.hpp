@@ -1,0 +1,433 @@
+// Vector kernels of the solvers: BLAS-1 updates, deterministic reductions, constraint
+// helpers and the geometric multigrid transfers.
+//
+// Reductions are two-stage with a fixed grid, so a dot product gives the same bits on
+// every run (the reference's MPI_Allreduce-based dots are deterministic for a fixed
+// rank count too).
+#pragma once
+
+#include "grid.hpp"
+
+#include <cmath>
+
+namespace ppf
+{
+  constexpr int reduce_threads = 256;
+  constexpr int reduce_blocks  = 1024;
+
+  template <typename T>
+  __device__ PPF_INLINE double
+  wave_sum(double v)
+  {
+    for (int off = warpSize / 2; off > 0; off /= 2)
+      v += __shfl_down(v, off, warpSize);
+    return v;
+  }
+
+  /// Block-level sum of one double per thread; valid in thread 0.
+  __device__ PPF_INLINE double
+  block_sum(double v)
+  {
+    __shared__ double partial[reduce_threads / 32];
+    v             = wave_sum<double>(v);
+    const int lane = threadIdx.x % warpSize;
+    const int wave = threadIdx.x / warpSize;
+    if (lane == 0)
+      partial[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+      for (int w = 0; w < int(blockDim.x) / warpSize; ++w)
+        r += partial[w];
+    return r;
+  }
+
+  template <typename T>
+  __global__ void
+  fill_kernel(T *__restrict__ x, T value, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      x[i] = value;
+  }
+
+  template <typename T, typename U>
+  __global__ void
+  convert_kernel(T *__restrict__ dst, const U *__restrict__ src, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      dst[i] = T(src[i]);
+  }
+
+  /// y = a*x + b*y
+  template <typename T>
+  __global__ void
+  axpby_kernel(T *__restrict__ y, T a, const T *__restrict__ x, T b, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      y[i] = a * x[i] + b * y[i];
+  }
+
+  /// z = x - y
+  template <typename T>
+  __global__ void
+  sub_kernel(T *__restrict__ z, const T *__restrict__ x, const T *__restrict__ y, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      z[i] = x[i] - y[i];
+  }
+
+  /// y = d .* x (diagonal preconditioner application)
+  template <typename T>
+  __global__ void
+  diag_mult_kernel(T *__restrict__ y, const T *__restrict__ d, const T *__restrict__ x,
+                   int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      y[i] = d[i] * x[i];
+  }
+
+  /// inv = 1/d with zero entries mapped to 1 (reference mf_operator.cc:98-100,245-255).
+  template <typename T>
+  __global__ void
+  invert_diagonal_kernel(T *__restrict__ inv, const T *__restrict__ d, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      inv[i] = d[i] == T(0) ? T(1) : T(1) / d[i];
+  }
+
+  /// Stage 1 of a dot product: one partial per block (grid-stride, fixed grid).
+  template <typename T>
+  __global__ void __launch_bounds__(reduce_threads)
+  dot_partial_kernel(double *__restrict__ partial, const T *__restrict__ x,
+                     const T *__restrict__ y, int64_t n)
+  {
+    double s = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+      s += double(x[i]) * double(y[i]);
+    s = block_sum(s);
+    if (threadIdx.x == 0)
+      partial[blockIdx.x] = s;
+  }
+
+  /// Stage 2: sums `count` partials in a fixed order into out[slot].
+  __global__ void __launch_bounds__(reduce_threads)
+  reduce_partials_kernel(double *__restrict__ out, const double *__restrict__ partial,
+                         int count)
+  {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x)
+      s += partial[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0)
+      out[0] = s;
+  }
+
+  /// CG update fused with the residual norm: x += alpha p, r += alpha Ap, partial |r|^2.
+  template <typename T>
+  __global__ void __launch_bounds__(reduce_threads)
+  cg_update_kernel(double *__restrict__ partial, T *__restrict__ x, T *__restrict__ r,
+                   const T *__restrict__ p, const T *__restrict__ ap, T alpha, int64_t n)
+  {
+    double s = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+      {
+        x[i] += alpha * p[i];
+        const T ri = r[i] + alpha * ap[i];
+        r[i]       = ri;
+        s += double(ri) * double(ri);
+      }
+    s = block_sum(s);
+    if (threadIdx.x == 0)
+      partial[blockIdx.x] = s;
+  }
+
+  /// p = -h + beta p
+  template <typename T>
+  __global__ void
+  cg_direction_kernel(T *__restrict__ p, const T *__restrict__ h, T beta, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      p[i] = -h[i] + beta * p[i];
+  }
+
+  /// Chebyshev step (Varga / Saad 12.1 recurrence as used by deal.II's
+  /// PreconditionChebyshev): d = c1 * d + c2 * P^{-1} (b - A x);  x += d.
+  /// With `first` set, d = c2 * P^{-1}(b - A x).
+  template <typename T>
+  __global__ void
+  chebyshev_step_kernel(T *__restrict__ x, T *__restrict__ d, const T *__restrict__ b,
+                        const T *__restrict__ ax, const T *__restrict__ inv_diag, T c1,
+                        T c2, int first, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+      return;
+    const T r  = b[i] - ax[i];
+    const T di = first ? c2 * inv_diag[i] * r : c1 * d[i] + c2 * inv_diag[i] * r;
+    d[i]       = di;
+    x[i] += di;
+  }
+
+  /// NaN/Inf probe of entry 0 (reference problem.cc:453-469 checks one entry per field).
+  template <typename T>
+  __global__ void
+  finite_probe_kernel(int *__restrict__ flag, const T *__restrict__ x)
+  {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+      flag[0] = isfinite(double(x[0])) ? 0 : 1;
+  }
+
+  /// Output diagnostics: partial sums of u^2 over deal.II's full node set (periodic
+  /// images counted with their multiplicity) and of u * jxw (images carry jxw = 0).
+  template <int dim, int degree, typename T>
+  __global__ void __launch_bounds__(reduce_threads)
+  norms_partial_kernel(double *__restrict__ partial, const GridDesc g, const T *__restrict__ u)
+  {
+    double s2 = 0.0, si = 0.0;
+    for (int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; node < g.n_nodes;
+         node += int64_t(gridDim.x) * blockDim.x)
+      {
+        int i[3];
+        i[0] = int(node % g.nodes[0]);
+        i[1] = int((node / g.nodes[0]) % g.nodes[1]);
+        i[2] = dim == 3 ? int(node / (int64_t(g.nodes[0]) * g.nodes[1])) : 0;
+        double mult = 1.0;
+        for (int d = 0; d < dim; ++d)
+          if (g.periodic[d] && i[d] == 0)
+            mult *= 2.0;
+        const double v = double(u[node]);
+        s2 += mult * v * v;
+        si += v * lumped_mass<dim, degree>(g, i);
+      }
+    s2 = block_sum(s2);
+    __syncthreads();
+    si = block_sum(si);
+    if (threadIdx.x == 0)
+      {
+        partial[blockIdx.x]             = s2;
+        partial[gridDim.x + blockIdx.x] = si;
+      }
+  }
+
+  /// dst = value at non-Dirichlet nodes, g at Dirichlet nodes.
+  template <int dim, typename T>
+  __global__ void
+  dirichlet_compose_kernel(T *__restrict__ dst, const GridDesc g, const T *__restrict__ gvals,
+                           unsigned int mask, T value, int keep_interior)
+  {
+    const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (node >= g.n_nodes)
+      return;
+    int i[3];
+    i[0] = int(node % g.nodes[0]);
+    i[1] = int((node / g.nodes[0]) % g.nodes[1]);
+    i[2] = dim == 3 ? int(node / (int64_t(g.nodes[0]) * g.nodes[1])) : 0;
+    if (is_dirichlet<dim>(g, i, mask))
+      dst[node] = gvals[node];
+    else if (!keep_interior)
+      dst[node] = value;
+  }
+
+  /// Zeroes the Dirichlet nodes of x (homogeneous level constraints).
+  template <int dim, typename T>
+  __global__ void
+  zero_dirichlet_kernel(T *__restrict__ x, const GridDesc g, unsigned int mask)
+  {
+    const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (node >= g.n_nodes)
+      return;
+    int i[3];
+    i[0] = int(node % g.nodes[0]);
+    i[1] = int((node / g.nodes[0]) % g.nodes[1]);
+    i[2] = dim == 3 ? int(node / (int64_t(g.nodes[0]) * g.nodes[1])) : 0;
+    if (is_dirichlet<dim>(g, i, mask))
+      x[node] = T(0);
+  }
+
+  // ------------------------------------------------------------------ transfers
+  //
+  // Nested uniform grids: the fine grid has twice the cells per direction.  Fine lattice
+  // coordinate I sits at coarse lattice position I/2.  Prolongation evaluates the coarse
+  // Q_p function at the fine nodes (exact embedding; this is the interpolation that
+  // deal.II's MGTransferGlobalCoarsening uses for h-coarsening); restriction is its
+  // transpose.  1-D weights:
+  //   p = 1: even I -> coarse I/2 (1); odd I -> (I-1)/2, (I+1)/2 (1/2, 1/2)
+  //   p = 2: even I -> coarse I/2 (1); odd I in coarse cell c = I/4 at xi = 1/4 or 3/4
+  //          -> local nodes (0,1,2) with (3/8, 3/4, -1/8) or (-1/8, 3/4, 3/8).
+
+  template <int degree>
+  PPF_HD PPF_INLINE int
+  prolongation_stencil_1d(const GridDesc &coarse, int d, int I, int (&J)[3], double (&w)[3])
+  {
+    if ((I & 1) == 0)
+      {
+        J[0] = I / 2;
+        w[0] = 1.0;
+        return 1;
+      }
+    if constexpr (degree == 1)
+      {
+        J[0] = (I - 1) / 2;
+        J[1] = wrap_lattice(coarse, d, (I + 1) / 2);
+        w[0] = 0.5;
+        w[1] = 0.5;
+        return 2;
+      }
+    else
+      {
+        const int c    = I / 4;
+        const int base = 2 * c;
+        J[0]           = base;
+        J[1]           = base + 1;
+        J[2]           = wrap_lattice(coarse, d, base + 2);
+        if ((I & 3) == 1)
+          {
+            w[0] = 3.0 / 8.0;
+            w[1] = 3.0 / 4.0;
+            w[2] = -1.0 / 8.0;
+          }
+        else
+          {
+            w[0] = -1.0 / 8.0;
+            w[1] = 3.0 / 4.0;
+            w[2] = 3.0 / 8.0;
+          }
+        return 3;
+      }
+  }
+
+  /// fine += P coarse; Dirichlet nodes of the fine level stay zero.
+  template <int dim, int degree, typename T>
+  __global__ void
+  prolongate_add_kernel(T *__restrict__ fine, const GridDesc gf, const T *__restrict__ coarse,
+                        const GridDesc gc, unsigned int mask)
+  {
+    const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (node >= gf.n_nodes)
+      return;
+    int I[3];
+    I[0] = int(node % gf.nodes[0]);
+    I[1] = int((node / gf.nodes[0]) % gf.nodes[1]);
+    I[2] = dim == 3 ? int(node / (int64_t(gf.nodes[0]) * gf.nodes[1])) : 0;
+    if (is_dirichlet<dim>(gf, I, mask))
+      return;
+    int    J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double w[3][3] = {{1, 0, 0}, {1, 0, 0}, {1, 0, 0}};
+    int    n[3]    = {1, 1, 1};
+    for (int d = 0; d < dim; ++d)
+      n[d] = prolongation_stencil_1d<degree>(gc, d, I[d], J[d], w[d]);
+    T sum = T(0);
+    for (int a2 = 0; a2 < n[2]; ++a2)
+      for (int a1 = 0; a1 < n[1]; ++a1)
+        for (int a0 = 0; a0 < n[0]; ++a0)
+          sum += T(w[0][a0] * w[1][a1] * w[2][a2]) *
+                 coarse[node_index(gc, J[0][a0], J[1][a1], J[2][a2])];
+    fine[node] += sum;
+  }
+
+  /// Transpose of prolongation_stencil_1d: fine lattice coordinates feeding coarse J.
+  template <int degree>
+  PPF_HD PPF_INLINE int
+  restriction_stencil_1d(const GridDesc &fine, int d, int J, int (&I)[5], double (&w)[5])
+  {
+    int    cand[5];
+    double cw[5];
+    int    m = 0;
+    if constexpr (degree == 1)
+      {
+        cand[m] = 2 * J, cw[m++] = 1.0;
+        cand[m] = 2 * J - 1, cw[m++] = 0.5;
+        cand[m] = 2 * J + 1, cw[m++] = 0.5;
+      }
+    else
+      {
+        if ((J & 1) == 0)
+          {
+            cand[m] = 2 * J, cw[m++] = 1.0;
+            cand[m] = 2 * J - 1, cw[m++] = 3.0 / 8.0;
+            cand[m] = 2 * J + 1, cw[m++] = 3.0 / 8.0;
+            cand[m] = 2 * J - 3, cw[m++] = -1.0 / 8.0;
+            cand[m] = 2 * J + 3, cw[m++] = -1.0 / 8.0;
+          }
+        else
+          {
+            cand[m] = 2 * J, cw[m++] = 1.0;
+            cand[m] = 2 * J - 1, cw[m++] = 3.0 / 4.0;
+            cand[m] = 2 * J + 1, cw[m++] = 3.0 / 4.0;
+          }
+      }
+    int n = 0;
+    for (int a = 0; a < m; ++a)
+      {
+        int i = cand[a];
+        if (fine.periodic[d])
+          {
+            const int period = fine.nodes[d];
+            i                = (i % period + period) % period;
+          }
+        else if (i < 0 || i >= fine.nodes[d])
+          continue;
+        I[n] = i;
+        w[n] = cw[a];
+        ++n;
+      }
+    return n;
+  }
+
+  /// coarse = P^T fine; Dirichlet nodes of the coarse level are zeroed.
+  template <int dim, int degree, typename T>
+  __global__ void
+  restrict_kernel(T *__restrict__ coarse, const GridDesc gc, const T *__restrict__ fine,
+                  const GridDesc gf, unsigned int mask)
+  {
+    const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (node >= gc.n_nodes)
+      return;
+    int J[3];
+    J[0] = int(node % gc.nodes[0]);
+    J[1] = int((node / gc.nodes[0]) % gc.nodes[1]);
+    J[2] = dim == 3 ? int(node / (int64_t(gc.nodes[0]) * gc.nodes[1])) : 0;
+    if (is_dirichlet<dim>(gc, J, mask))
+      {
+        coarse[node] = T(0);
+        return;
+      }
+    int    I[3][5] = {};
+    double w[3][5] = {{1}, {1}, {1}};
+    int    n[3]    = {1, 1, 1};
+    for (int d = 0; d < dim; ++d)
+      n[d] = restriction_stencil_1d<degree>(gf, d, J[d], I[d], w[d]);
+    T sum = T(0);
+    for (int a2 = 0; a2 < n[2]; ++a2)
+      for (int a1 = 0; a1 < n[1]; ++a1)
+        for (int a0 = 0; a0 < n[0]; ++a0)
+          sum += T(w[0][a0] * w[1][a1] * w[2][a2]) *
+                 fine[node_index(gf, I[0][a0], I[1][a1], I[2][a2])];
+    coarse[node] = sum;
+  }
+
+  /// Nodal injection fine -> coarse (coarse lattice J = fine lattice 2J); transfers
+  /// non-operand fields to the levels (reference group_solution_handler.h:356-417).
+  template <int dim, typename T, typename U>
+  __global__ void
+  inject_kernel(T *__restrict__ coarse, const GridDesc gc, const U *__restrict__ fine,
+                const GridDesc gf)
+  {
+    const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (node >= gc.n_nodes)
+      return;
+    const int j0 = int(node % gc.nodes[0]);
+    const int j1 = int((node / gc.nodes[0]) % gc.nodes[1]);
+    const int j2 = dim == 3 ? int(node / (int64_t(gc.nodes[0]) * gc.nodes[1])) : 0;
+    coarse[node] = T(fine[node_index(gf, 2 * j0, 2 * j1, 2 * j2)]);
+  }
+} // namespace ppf

@@ -1,0 +1,168 @@
+"""Parameter files (PRISMS-PF ``.prm`` text) of the benchmark configurations.
+
+`config_prm(name, **overrides)` returns the text for one of the five configurations
+with optional size overrides (used by the tests to run the same model on small
+meshes).  Keys follow the reference's parameter names
+(src/user_inputs/{spatial_discretization,temporal_discretization,solve_parameters,
+constraint_parameters,io_parameters,miscellaneous_parameters}.cc).
+"""
+
+from __future__ import annotations
+
+# name -> (application library, description)
+CONFIGS = {
+    "allen_cahn_2d": ("allen_cahn_2d_q1", "Allen-Cahn 2D explicit, Q1, 256^2 cells on [0,100]^2"),
+    "cahn_hilliard_3d": ("cahn_hilliard_3d_q2", "Cahn-Hilliard 3D explicit, Q2, 64^3 cells on [0,64]^3"),
+    "poisson_3d": ("poisson_3d_q2", "Poisson 3D, Q2, 128^3 cells on the unit cube, CG + GMG"),
+    "allen_cahn_adjoint_3d": ("allen_cahn_3d_q1", "Allen-Cahn 3D adjoint, Q1, 128^3 cells on [0,128]^3"),
+    "precipitate_3d": ("precipitate_3d_q1", "Coupled AC/CH precipitate 3D, Q1, 192^3 cells on [0,192]^3"),
+}
+
+
+def mesh_block(dim, size, refinement, subdivisions=None, periodic=True, lower=None):
+    axes = "xyz"[:dim]
+    subdivisions = subdivisions or [1] * dim
+    lower = lower or [0.0] * dim
+    size = size if isinstance(size, (list, tuple)) else [size] * dim
+    lines = ["subsection Rectangular mesh"]
+    for d, a in enumerate(axes):
+        lines.append(f"  set {a} size = {size[d]!r}")
+        if lower[d]:
+            lines.append(f"  set {a} lower bound = {lower[d]!r}")
+        lines.append(f"  set {a} subdivisions = {subdivisions[d]}")
+        lines.append(f"  set {a} periodic = {'true' if periodic else 'false'}")
+    lines.append("end")
+    lines.append(f"set global refinement = {refinement}")
+    return "\n".join(lines)
+
+
+def _common(steps, dt, outputs=1, seed=2025):
+    return "\n".join([
+        f"set time step = {dt!r}",
+        f"set number steps = {steps}",
+        "subsection output",
+        "  set condition = EQUAL_SPACING",
+        f"  set number = {outputs}",
+        "end",
+        f"set random seed = {seed}",
+    ])
+
+
+def allen_cahn(dim=2, size=100.0, refinement=8, steps=1000, dt=0.005, L=1.0, kappa=1.5,
+               subdivisions=None, outputs=1, n_circles=8):
+    return "\n".join([
+        mesh_block(dim, size, refinement, subdivisions),
+        _common(steps, dt, outputs),
+        f"set Model constant L = {L!r}, DOUBLE",
+        f"set Model constant kappa = {kappa!r}, DOUBLE",
+        f"set Model constant n_circles = {n_circles}, INT",
+        "set Model constant radius_min = 5.0, DOUBLE",
+        "set Model constant radius_max = 15.0, DOUBLE",
+        "set Model constant interface_width = 1.0, DOUBLE",
+        "set Model constant noise_amplitude = 0.05, DOUBLE",
+    ]) + "\n"
+
+
+def cahn_hilliard(size=64.0, refinement=6, steps=200, dt=1e-4, M=1.0, kappa=1.0, outputs=1,
+                  subdivisions=None):
+    return "\n".join([
+        mesh_block(3, size, refinement, subdivisions),
+        _common(steps, dt, outputs),
+        f"set Model constant M = {M!r}, DOUBLE",
+        f"set Model constant kappa = {kappa!r}, DOUBLE",
+        "set Model constant c0 = 0.5, DOUBLE",
+        "set Model constant noise_amplitude = 0.01, DOUBLE",
+    ]) + "\n"
+
+
+def poisson(dim=3, refinement=7, mg_depth=7, tolerance=1e-10, preconditioner="GMG",
+            smoother_degree=4, max_iterations=1000, level_precision="double",
+            coarse_solver="cg", size=1.0, tolerance_type="AbsoluteResidual"):
+    faces = ", ".join(["Dirichlet"] * (2 * dim))
+    return "\n".join([
+        mesh_block(dim, size, refinement, periodic=False),
+        _common(0, 0.0),
+        "subsection boundary conditions: 0",
+        "  set variables = u",
+        f"  set conditions = {faces}",
+        "end",
+        "subsection linear solver parameters: 0",
+        "  set solver_ids = 1",
+        f"  set tolerance type = {tolerance_type}",
+        f"  set tolerance value = {tolerance!r}",
+        f"  set max iterations = {max_iterations}",
+        f"  set preconditioner = {preconditioner}",
+        f"  set mg depth = {mg_depth}",
+        f"  set mg level precision = {level_precision}",
+        f"  set mg coarse solver = {coarse_solver}",
+        "  subsection Chebyshev",
+        f"    set smoother degree = {smoother_degree}",
+        "    set smoothing range = 15.0",
+        "    set eigenvalue cg iterations = 10",
+        "  end",
+        "end",
+    ]) + "\n"
+
+
+def precipitate(size=192.0, refinement=6, subdivisions=(3, 3, 3), steps=1000, dt=5e-3,
+                outputs=1, n_nuclei=20, radius=4.0):
+    return "\n".join([
+        mesh_block(3, size, refinement, list(subdivisions)),
+        _common(steps, dt, outputs),
+        "set Model constant W = 1.0, DOUBLE",
+        "set Model constant M = 1.0, DOUBLE",
+        "set Model constant L = 1.0, DOUBLE",
+        "set Model constant kappa_c = 1.0, DOUBLE",
+        "set Model constant kappa_eta = 1.0, DOUBLE",
+        "set Model constant c0 = 0.3, DOUBLE",
+        "set Model constant noise_amplitude = 0.01, DOUBLE",
+        f"set Model constant n_nuclei = {n_nuclei}, INT",
+        f"set Model constant nucleus_radius = {radius!r}, DOUBLE",
+    ]) + "\n"
+
+
+def testbed(dim=2, refinement=3, size=4.0, steps=2, dt=1e-3, dirichlet_faces=(0, 3),
+            tolerance=1e-10, preconditioner="None", mg_depth=2, periodic=False):
+    conds = ["Natural"] * (2 * dim)
+    for f in dirichlet_faces:
+        conds[f] = "Dirichlet"
+    return "\n".join([
+        mesh_block(dim, size, refinement, periodic=periodic),
+        _common(steps, dt, 1),
+        "subsection boundary conditions: 0",
+        "  set variables = c, v",
+        f"  set conditions = {', '.join(conds)}",
+        "end",
+        "subsection linear solver parameters: 0",
+        "  set solver_ids = 1",
+        "  set tolerance type = AbsoluteResidual",
+        f"  set tolerance value = {tolerance!r}",
+        "  set max iterations = 500",
+        f"  set preconditioner = {preconditioner}",
+        f"  set mg depth = {mg_depth}",
+        "  set mg coarse solver = cg",
+        "end",
+        "set Model constant D0 = 0.7, DOUBLE",
+        "set Model constant alpha = 0.5, DOUBLE",
+        "set Model constant amplitude = 0.3, DOUBLE",
+    ]) + "\n"
+
+
+def config_prm(name: str, **kw) -> str:
+    """Parameter text of a benchmark configuration (keyword overrides shrink it)."""
+    if name == "allen_cahn_2d":
+        return allen_cahn(dim=2, **kw)
+    if name == "cahn_hilliard_3d":
+        return cahn_hilliard(**kw)
+    if name == "poisson_3d":
+        return poisson(**kw)
+    if name == "allen_cahn_adjoint_3d":
+        kw.setdefault("size", 128.0)
+        kw.setdefault("refinement", 7)
+        kw.setdefault("steps", 500)
+        kw.setdefault("dt", 0.01)
+        kw.setdefault("kappa", 1.5)
+        return allen_cahn(dim=3, **kw)
+    if name == "precipitate_3d":
+        return precipitate(**kw)
+    raise KeyError(name)

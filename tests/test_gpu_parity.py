@@ -1,0 +1,253 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle on identical inputs.
+
+Bars: operator application within 1e-12 relative in FP64 (1e-5 in FP32); explicit runs
+within 1e-10 after several increments; CG iteration counts within +-1 of the oracle with
+the same stopping rule; adjoint quantities within 1e-10 of the oracle's FP64 adjoint.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from phasefield_amd import Context, prm
+from phasefield_amd.capi import (FP32, FP64, OP_LHS, OP_LHS_PLAIN, OP_RHS, SCALE_INVM,
+                                 SCALE_INVM_SQRT, SCALE_NONE)
+from oracle.binding import Oracle
+from tests.helpers import (block_to_full, block_to_stored, rel_err, to_full, to_stored)
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a, dtype=None):
+    return torch.tensor(np.asarray(a), dtype=dtype or torch.float64, device="cuda")
+
+
+def seed_fields(ctx: Context, orc: Oracle, rng, lo=0.0, hi=1.0):
+    """Same random state (every field, every age) on both sides; images = masters."""
+    info = ctx.info
+    for f in range(info.n_fields):
+        age = 0
+        while True:
+            vals = to_full(rng.uniform(lo, hi, info.n_nodes), info)
+            try:
+                ctx.set_field(f, vals, age)
+            except Exception:
+                break
+            orc.set_field(f, vals, age)
+            age += 1
+
+
+# (app, prm text, [(block_id, which, n_block)])
+OPERATOR_CASES = [
+    ("allen_cahn_2d_q1", prm.allen_cahn(dim=2, size=6.0, refinement=2, subdivisions=[2, 3]),
+     [(1, OP_RHS, 1), (2, OP_RHS, 1)]),
+    ("allen_cahn_3d_q1", prm.allen_cahn(dim=3, size=4.0, refinement=2), [(1, OP_RHS, 1)]),
+    ("cahn_hilliard_3d_q2", prm.cahn_hilliard(size=4.0, refinement=2),
+     [(0, OP_RHS, 1), (1, OP_RHS, 1), (2, OP_RHS, 1)]),
+    ("precipitate_3d_q1", prm.precipitate(size=8.0, refinement=2, subdivisions=(1, 1, 1)),
+     [(0, OP_RHS, 1), (1, OP_RHS, 1), (2, OP_RHS, 1)]),
+    ("poisson_2d_q2", prm.poisson(dim=2, refinement=3, mg_depth=2),
+     [(1, OP_RHS, 1), (1, OP_LHS, 1), (1, OP_LHS_PLAIN, 1)]),
+    ("poisson_3d_q2", prm.poisson(dim=3, refinement=2, mg_depth=2),
+     [(1, OP_RHS, 1), (1, OP_LHS, 1), (1, OP_LHS_PLAIN, 1)]),
+    ("testbed_2d_q2", prm.testbed(dim=2, refinement=3),
+     [(0, OP_RHS, 1), (1, OP_RHS, 1), (1, OP_LHS, 1), (1, OP_LHS_PLAIN, 1), (2, OP_RHS, 1)]),
+    ("testbed_3d_q1", prm.testbed(dim=3, refinement=2, dirichlet_faces=(1, 4)),
+     [(0, OP_RHS, 1), (1, OP_LHS, 1), (1, OP_LHS_PLAIN, 1), (2, OP_RHS, 1)]),
+]
+
+
+@pytest.mark.parametrize("app,text,ops", OPERATOR_CASES, ids=[c[0] for c in OPERATOR_CASES])
+@pytest.mark.parametrize("path", ["auto", "generic"])
+def test_operator_matches_oracle(app, text, ops, path):
+    ctx, orc = Context(app, text), Oracle(app, text)
+    info = ctx.info
+    rng = np.random.default_rng(7)
+    seed_fields(ctx, orc, rng)
+    ctx.set_time(3, 0.37)
+    orc.set_time(3, 0.37)
+    ctx.set_option("operator_path", path)
+    for block_id, which, nb in ops:
+        for scaling in (SCALE_NONE, SCALE_INVM, SCALE_INVM_SQRT):
+            src_stored = rng.uniform(-1, 1, nb * info.n_nodes)
+            src_full = block_to_full(src_stored, info, nb)
+            ref = orc.apply_operator(block_id, which, scaling, src_full, nb)
+            d_src = dev(src_stored)
+            d_dst = torch.zeros_like(d_src)
+            ctx.apply_operator(block_id, which, scaling, FP64, d_src.data_ptr(), d_dst.data_ptr())
+            got = d_dst.cpu().numpy()
+            err = rel_err(got, block_to_stored(ref, info, nb))
+            print(app, path, block_id, which, scaling, err, flush=True)
+            assert err <= 1e-12, (block_id, which, scaling, err)
+
+
+@pytest.mark.parametrize("app,text,ops", OPERATOR_CASES[2:6], ids=[c[0] for c in OPERATOR_CASES[2:6]])
+def test_operator_fp32_matches_oracle(app, text, ops):
+    ctx, orc = Context(app, text), Oracle(app, text)
+    info = ctx.info
+    rng = np.random.default_rng(11)
+    seed_fields(ctx, orc, rng)
+    for block_id, which, nb in ops:
+        src_stored = rng.uniform(-1, 1, nb * info.n_nodes)
+        ref = orc.apply_operator(block_id, which, SCALE_NONE, block_to_full(src_stored, info, nb), nb)
+        d_src = dev(src_stored, torch.float32)
+        d_dst = torch.zeros_like(d_src)
+        ctx.apply_operator(block_id, which, SCALE_NONE, FP32, d_src.data_ptr(), d_dst.data_ptr())
+        err = rel_err(d_dst.double().cpu().numpy(), block_to_stored(ref, info, nb))
+        assert err <= 1e-5, (block_id, which, err)
+
+
+DIAGONAL_CASES = [
+    ("poisson_2d_q2", prm.poisson(dim=2, refinement=3, mg_depth=2), 1),
+    ("poisson_3d_q2", prm.poisson(dim=3, refinement=2, mg_depth=2), 1),
+    ("testbed_2d_q2", prm.testbed(dim=2, refinement=3), 1),
+    ("testbed_3d_q1", prm.testbed(dim=3, refinement=2), 1),
+]
+
+
+@pytest.mark.parametrize("app,text,block_id", DIAGONAL_CASES, ids=[c[0] for c in DIAGONAL_CASES])
+@pytest.mark.parametrize("scaling", [SCALE_NONE, SCALE_INVM_SQRT])
+def test_inverse_diagonal_matches_oracle(app, text, block_id, scaling):
+    ctx, orc = Context(app, text), Oracle(app, text)
+    info = ctx.info
+    seed_fields(ctx, orc, np.random.default_rng(3))
+    ref = orc.inverse_diagonal(block_id, scaling)
+    d = torch.zeros(info.n_nodes, dtype=torch.float64, device="cuda")
+    ctx.inverse_diagonal(block_id, scaling, FP64, d.data_ptr())
+    assert rel_err(d.cpu().numpy(), to_stored(ref, info)) <= 1e-12
+
+
+RUN_CASES = [
+    ("allen_cahn_2d_q1", prm.allen_cahn(dim=2, size=16.0, refinement=4, steps=20, dt=0.01, outputs=4), 20),
+    ("cahn_hilliard_3d_q2", prm.cahn_hilliard(size=8.0, refinement=2, steps=10, dt=1e-3, outputs=2), 10),
+    ("precipitate_3d_q1", prm.precipitate(size=16.0, refinement=3, subdivisions=(1, 1, 1), steps=10, outputs=2), 10),
+    ("testbed_2d_q2", prm.testbed(dim=2, refinement=3, steps=5), 5),
+    ("testbed_3d_q1", prm.testbed(dim=3, refinement=2, steps=3), 3),
+]
+
+
+@pytest.mark.parametrize("app,text,steps", RUN_CASES, ids=[c[0] for c in RUN_CASES])
+def test_run_matches_oracle(app, text, steps):
+    """Whole increment loop (ICs, blocks in order, postprocess, rotation)."""
+    ctx, orc = Context(app, text), Oracle(app, text)
+    assert ctx.run(steps) == 0
+    assert orc.run(steps) == 0
+    for f in range(ctx.info.n_fields):
+        got, ref = ctx.get_field(f).ravel(), orc.get_field(f)
+        err = rel_err(got, ref)
+        print(app, "field", f, err, flush=True)
+        assert err <= 1e-10, (f, err)
+    # Output diagnostics: l2-norm over deal.II's node set and integral.
+    for f in range(ctx.info.n_fields):
+        a, b = ctx.field_norms(f), orc.field_norms(f)
+        assert abs(a[0] - b[0]) <= 1e-10 * max(1.0, abs(b[0]))
+        assert abs(a[1] - b[1]) <= 1e-10 * max(1.0, abs(b[1]))
+
+
+LINEAR_CASES = [
+    ("poisson_2d_q2", dict(dim=2, refinement=4, preconditioner="None", mg_depth=1)),
+    ("poisson_2d_q2", dict(dim=2, refinement=4, preconditioner="Chebyshev", mg_depth=1)),
+    ("poisson_2d_q2", dict(dim=2, refinement=4, preconditioner="GMG", mg_depth=3, coarse_solver="smoother")),
+    ("poisson_2d_q2", dict(dim=2, refinement=4, preconditioner="GMG", mg_depth=4, coarse_solver="cg")),
+    ("poisson_3d_q2", dict(dim=3, refinement=3, preconditioner="GMG", mg_depth=3, coarse_solver="cg")),
+    ("poisson_2d_q2", dict(dim=2, refinement=4, preconditioner="None", tolerance_type="RMSEPerField", size=2.0)),
+]
+
+
+@pytest.mark.parametrize("app,kw", LINEAR_CASES)
+def test_linear_solve_matches_oracle(app, kw):
+    text = prm.poisson(tolerance=1e-9, **kw)
+    ctx, orc = Context(app, text), Oracle(app, text)
+    ctx.run(0)
+    orc.run(0)
+    st = ctx.last_solve_stats(1)
+    ref = orc.last_stats if hasattr(orc, "last_stats") else None
+    orc2 = Oracle(app, text)
+    orc2.solve_block(0)
+    ost = orc2.linear_solve(1)
+    print(app, kw, "gpu its", st.iterations, "oracle its", ost["iterations"], flush=True)
+    assert st.converged and ost["converged"]
+    assert abs(st.iterations - ost["iterations"]) <= 1
+    u_gpu, u_ref = ctx.get_field(0).ravel(), orc.get_field(0)
+    assert rel_err(u_gpu, u_ref) <= 1e-6
+
+
+def test_testbed_gmg_variable_coefficient(device):
+    """Generic operator, generic diagonal, non-src dependency injected to the levels."""
+    text = prm.testbed(dim=2, refinement=4, preconditioner="GMG", mg_depth=3, steps=0)
+    ctx, orc = Context("testbed_2d_q2", text), Oracle("testbed_2d_q2", text)
+    ctx.run(0)
+    orc.run(0)
+    st = ctx.last_solve_stats(1)
+    orc2 = Oracle("testbed_2d_q2", text)
+    orc2.solve_block(0)
+    ost = orc2.linear_solve(1)
+    assert st.converged and abs(st.iterations - ost["iterations"]) <= 1
+    assert rel_err(ctx.get_field(1).ravel(), orc.get_field(1)) <= 1e-7
+
+
+def test_mixed_precision_gmg_reaches_fp64_tolerance(device):
+    text = prm.poisson(dim=3, refinement=3, mg_depth=3, tolerance=1e-9, level_precision="single")
+    ctx = Context("poisson_3d_q2", text)
+    ctx.run(0)
+    st = ctx.last_solve_stats(1)
+    orc = Oracle("poisson_3d_q2", prm.poisson(dim=3, refinement=3, mg_depth=3, tolerance=1e-9))
+    orc.run(0)
+    assert st.converged and st.final_residual <= st.tolerance
+    assert rel_err(ctx.get_field(0).ravel(), orc.get_field(0)) <= 1e-6
+
+
+ADJ_TEXT = prm.allen_cahn(dim=3, size=8.0, refinement=3, steps=0, dt=0.01, kappa=1.5)
+
+
+@pytest.mark.parametrize("precision", [FP64])
+def test_adjoint_matches_oracle(device, precision):
+    ctx, orc = Context("allen_cahn_3d_q1", ADJ_TEXT), Oracle("allen_cahn_3d_q1", ADJ_TEXT)
+    info = ctx.info
+    rng = np.random.default_rng(5)
+    phi0 = rng.uniform(-1, 1, info.n_nodes)
+    lam = rng.uniform(-1, 1, info.n_nodes)
+    n, L, kappa = 12, 1.1, 1.3
+    d_phi0, d_out = dev(phi0), torch.zeros(info.n_nodes, dtype=torch.float64, device="cuda")
+    ctx.adjoint_forward(d_phi0.data_ptr(), L, kappa, n, precision, d_out.data_ptr())
+    ref = orc.adjoint_forward(to_full(phi0, info), L, kappa, n)
+    assert rel_err(d_out.cpu().numpy(), to_stored(ref, info)) <= 1e-12
+    for every in (1, 5, 12, 50):
+        d_lam, d_g = dev(lam), torch.zeros(info.n_nodes, dtype=torch.float64, device="cuda")
+        dl, dk = ctx.adjoint_vjp(d_phi0.data_ptr(), d_lam.data_ptr(), L, kappa, n, every,
+                                 precision, d_g.data_ptr())
+        lam_full = np.zeros(info.n_dofs)
+        lam_full[:] = to_full(lam, info)
+        # The oracle folds image entries onto masters: pass zeros at images.
+        lam_masters = to_stored(lam_full, info)
+        rdl, rdk, rg = orc.adjoint_vjp(to_full(phi0, info), _zero_images(lam_masters, info), L, kappa, n)
+        assert abs(dl - rdl) <= 1e-10 * abs(rdl)
+        assert abs(dk - rdk) <= 1e-10 * abs(rdk)
+        assert rel_err(d_g.cpu().numpy(), to_stored(rg, info)) <= 1e-10
+
+
+def _zero_images(stored, info):
+    full = to_full(stored, info).reshape(tuple(reversed([info.degree * info.cells[d] + 1 for d in range(info.dim)])))
+    for ax in range(info.dim):
+        d = info.dim - 1 - ax
+        if info.periodic[d]:
+            idx = [slice(None)] * info.dim
+            idx[ax] = -1
+            full[tuple(idx)] = 0.0
+    return full.ravel()
+
+
+def test_explicit_matches_adjoint_forward(device):
+    """The adjoint module's forward step equals the application's explicit block."""
+    text = prm.allen_cahn(dim=3, size=8.0, refinement=3, steps=4, dt=0.01, L=1.0, kappa=1.5)
+    ctx = Context("allen_cahn_3d_q1", text)
+    ctx.run(0)
+    phi0 = to_stored(ctx.get_field(0).ravel(), ctx.info)
+    ctx.run(4)
+    phi4 = to_stored(ctx.get_field(0).ravel(), ctx.info)
+    d0, d4 = dev(phi0), torch.zeros(ctx.info.n_nodes, dtype=torch.float64, device="cuda")
+    ctx.adjoint_forward(d0.data_ptr(), 1.0, 1.5, 4, FP64, d4.data_ptr())
+    assert rel_err(d4.cpu().numpy(), phi4) <= 1e-12
