@@ -103,9 +103,12 @@ int ppf_field_name(ppf_ctx *ctx, int field, char *buf, int len);
  * linear block, overriding the parameter file). */
 int ppf_set_option(ppf_ctx *ctx, const char *key, const char *value);
 
-/* Field state `age` (0 = current, k = old_k) from/to host memory over deal.II's node
- * set (replaces SolutionIndexer / GroupSolutionHandler access,
- * src/core/solution_indexer.cc:12-104). */
+/* Field state `age` (0 = current, k = old_k, PPF_AGE_LATEST = the newest values) from/to
+ * host memory over deal.II's node set (replaces SolutionIndexer / GroupSolutionHandler
+ * access, src/core/solution_indexer.cc:12-104).  Every increment ends with the old-state
+ * rotation, after which the newest values of a field with old states are in old_1, as in
+ * the reference; PPF_AGE_LATEST resolves that. */
+#define PPF_AGE_LATEST -1
 int ppf_set_field(ppf_ctx *ctx, int field, int age, const double *host, long long n);
 int ppf_get_field(ppf_ctx *ctx, int field, int age, double *host, long long n);
 

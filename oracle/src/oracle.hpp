@@ -203,7 +203,7 @@ namespace oracle
       top   = make_mesh(inputs.spatial_discretization.global_refinement);
       for (std::size_t f = 0; f < fields.size(); ++f)
         cons.push_back(make_constraints(top, int(f), false));
-      jxw = lumped_mass(top, cons);
+      jxw = lumped_mass(top);
       // storage: 1 + oldest age read
       std::vector<unsigned> max_age(fields.size(), 0);
       for (const auto &b : blocks)
@@ -565,13 +565,38 @@ namespace oracle
         }
     }
 
-    /// jxw of every DoF: integral of its basis function, constrained DoFs zero.
+    /// jxw of every DoF: the integral of its basis function, assembled with the
+    /// periodicity constraints only (the reference's "generic" constraints,
+    /// src/core/constraint_manager.cc:152-173), so images carry zero and Dirichlet DoFs
+    /// keep their mass.
     std::vector<double>
-    lumped_mass(const Mesh &m, const std::vector<Constraints> &cs) const
+    lumped_mass(const Mesh &m) const
     {
-      std::vector<double>   out(m.ndofs);
-      std::vector<double *> dst {out.data()};
-      Inputs                in;
+      Constraints periodic;
+      periodic.kind.assign(m.ndofs, free_dof);
+      periodic.target.assign(m.ndofs, -1);
+      periodic.g.assign(m.ndofs, 0.0);
+      for (int64_t dof = 0; dof < m.ndofs; ++dof)
+        {
+          int i[3];
+          m.coords(dof, i);
+          bool img = false;
+          for (int d = 0; d < dim; ++d)
+            if (m.periodic[d] && i[d] == m.N[d] - 1)
+              {
+                img  = true;
+                i[d] = 0;
+              }
+          if (img)
+            {
+              periodic.kind[dof]   = image_dof;
+              periodic.target[dof] = m.index(i[0], i[1], i[2]);
+            }
+        }
+      std::vector<double>      out(m.ndofs);
+      std::vector<double *>    dst {out.data()};
+      std::vector<Constraints> cs(PPF_MAX_FIELDS, periodic);
+      Inputs                   in;
       cell_loop(m, cs, in, {0}, dst, [](FC &fc) { fc.value_term[0] = 1.0; });
       return out;
     }
@@ -1061,7 +1086,7 @@ namespace oracle
           L.mesh  = make_mesh(unsigned(topr - depth + 1 + i));
           for (std::size_t f = 0; f < fields.size(); ++f)
             L.cons.push_back(make_constraints(L.mesh, int(f), true));
-          L.jxw = lumped_mass(L.mesh, L.cons);
+          L.jxw = lumped_mass(L.mesh);
           // inject non-src fields
           L.fields.resize(fields.size());
           for (std::size_t f = 0; f < fields.size(); ++f)
@@ -1302,15 +1327,30 @@ namespace oracle
         for (int f : block_fields(b))
           for (std::size_t a = 1; a < storage[f].size(); ++a)
             storage[f][a] = storage[f][0];
+      for (int f : block_fields(b))
+        rotated[f] = false;
     }
 
     void
     rotate(int b)
     {
       for (int f : block_fields(b))
-        for (int a = int(storage[f].size()) - 1; a >= 1; --a)
-          std::swap(storage[f][a], storage[f][a - 1]);
+        {
+          for (int a = int(storage[f].size()) - 1; a >= 1; --a)
+            std::swap(storage[f][a], storage[f][a - 1]);
+          rotated[f] = storage[f].size() > 1;
+        }
     }
+
+    /// Age holding the newest values of field f: after a rotation (the end of every
+    /// increment) that is old_1, as in the reference's solution handler.
+    int
+    latest_age(int f) const
+    {
+      return rotated.count(f) && rotated.at(f) ? 1 : 0;
+    }
+
+    std::map<int, bool> rotated;
 
     std::pair<double, double>
     norms(int f, int age) const

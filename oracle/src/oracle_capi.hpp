@@ -138,6 +138,8 @@ int
 oracle_get_field(void *h, int f, int age, double *v, long long n)
 {
   return oracle::guard(h, [&](auto &m) {
+    if (age < 0 && f >= 0 && f < int(m.storage.size()))
+      age = m.latest_age(f);
     if (n != m.top.ndofs || f < 0 || f >= int(m.storage.size()) || age < 0 ||
         age >= int(m.storage[f].size()))
       throw std::invalid_argument("oracle_get_field: bad field, age or size");
@@ -226,6 +228,8 @@ int
 oracle_field_norms(void *h, int f, int age, double *l2, double *integral)
 {
   return oracle::guard(h, [&](auto &m) {
+    if (age < 0)
+      age = m.latest_age(f);
     auto r    = m.norms(f, age);
     *l2       = r.first;
     *integral = r.second;

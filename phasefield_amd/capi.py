@@ -21,6 +21,7 @@ OK = 0
 OP_RHS, OP_LHS, OP_LHS_PLAIN = 0, 1, 2
 SCALE_NONE, SCALE_INVM, SCALE_INVM_SQRT = 0, 1, 2
 FP64, FP32 = 0, 1
+AGE_LATEST = -1
 
 # Every symbol include/ppf.h declares.
 EXPORTED_SYMBOLS = (

@@ -293,6 +293,8 @@ namespace ppf
     double *
     field_ptr(int f, int age)
     {
+      if (age == -1 && f >= 0 && f < int(fields.size()))
+        age = latest_age(f);
       if (f < 0 || f >= int(fields.size()) || age < 0 || age >= int(storage[f].size()))
         throw Error(status_invalid, "field/age out of range");
       return storage[f][age].get();
@@ -931,21 +933,23 @@ namespace ppf
     /// Lanczos estimate of the largest eigenvalue of P^{-1} A from `its` CG steps (the
     /// approach of deal.II's PreconditionChebyshev); lmax gets a 1.2 safety factor and
     /// lmin = lmax / smoothing_range.  The start vector is a fixed pseudo-random vector
-    /// with zero Dirichlet rows (parity with deal.II's start vector is not claimed).
+    /// (parity with deal.II's own start vector is not claimed).
     template <typename T, class Vmult>
     void
     estimate_eigenvalues(Vmult &&A, const T *inv_diag, int64_t n,
                          const prismspf::ChebyshevParameters &cp, double &lmax, double &lmin,
                          T *r, T *p, T *ap, T *z)
     {
+      // Start vector: splitmix64 of the block-vector index (field-major, stored node
+      // order), uniform in [-0.5, 0.5).
       std::vector<T> host(n);
-      uint64_t       s = 0x243f6a8885a308d3ull;
       for (int64_t i = 0; i < n; ++i)
         {
-          s ^= s << 13;
-          s ^= s >> 7;
-          s ^= s << 17;
-          host[i] = T(double(s >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+          uint64_t x = uint64_t(i) + 0x9e3779b97f4a7c15ull;
+          x          = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+          x          = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+          x ^= x >> 31;
+          host[i] = T(double(x >> 11) * (1.0 / 9007199254740992.0) - 0.5);
         }
       PPF_HIP_CHECK(hipMemcpyAsync(z, host.data(), sizeof(T) * n, hipMemcpyHostToDevice, stream));
       // CG for A e = 0 from the pseudo-random guess: the initial residual is A x0, which
@@ -1159,6 +1163,7 @@ namespace ppf
           for (auto &l : ls.levels)
             {
               const bool want_f = l.precision == 1;
+              const int  stride = fine.cells[0] / l.grid.cells[0];
               auto      &vd     = l.field_d[in.field];
               auto      &vf     = l.field_f[in.field];
               if (vd.size() <= std::size_t(age))
@@ -1168,7 +1173,7 @@ namespace ppf
               vd[age].resize(l.grid.n_nodes);
               hipLaunchKernelGGL((inject_kernel<dim, double, double>),
                                  dim3(grid_for(l.grid.n_nodes, 256)), dim3(256), 0, stream,
-                                 vd[age].get(), l.grid, field_ptr(in.field, age), fine);
+                                 vd[age].get(), l.grid, field_ptr(in.field, age), fine, stride);
               if (want_f)
                 {
                   vf[age].resize(l.grid.n_nodes);
@@ -1355,6 +1360,8 @@ namespace ppf
         throw Error(status_unsupported, "Newton solve blocks are not supported yet");
       if (timer.get_increment() == 0)
         copy_current_to_old(b);
+      for (auto f : blocks[b].field_indices)
+        rotated[f] = false;
     }
 
     /// Old-state rotation by pointer swaps (reference group_solution_handler.cc:369-403).
@@ -1362,8 +1369,20 @@ namespace ppf
     rotate(int b)
     {
       for (auto f : blocks[b].field_indices)
-        for (int age = int(storage[f].size()) - 1; age >= 1; --age)
-          storage[f][age].swap(storage[f][age - 1]);
+        {
+          for (int age = int(storage[f].size()) - 1; age >= 1; --age)
+            storage[f][age].swap(storage[f][age - 1]);
+          rotated[f] = storage[f].size() > 1;
+        }
+    }
+
+    /// Age holding the newest values of field f.  Every increment ends with the
+    /// rotation, after which the reference's solution handler keeps the newest values
+    /// in old_1 (the current slot is then overwritten by the next solve).
+    int
+    latest_age(int f) const
+    {
+      return rotated[f] ? 1 : 0;
     }
 
     /// Problem::solve_increment (reference problem.cc:414-558).  Returns the exit status:
@@ -1477,5 +1496,6 @@ namespace ppf
     DeviceBuffer<double>                           reduce_buf;
     DeviceBuffer<double>                           scalar_buf;
     std::string                                    log;
+    std::array<bool, PPF_MAX_FIELDS>               rotated {};
   };
 } // namespace ppf

@@ -415,12 +415,13 @@ namespace ppf
     coarse[node] = sum;
   }
 
-  /// Nodal injection fine -> coarse (coarse lattice J = fine lattice 2J); transfers
-  /// non-operand fields to the levels (reference group_solution_handler.h:356-417).
+  /// Nodal injection fine -> coarse (coarse lattice J = fine lattice stride*J, stride a
+  /// power of two); transfers non-operand fields to the levels
+  /// (reference group_solution_handler.h:356-417).
   template <int dim, typename T, typename U>
   __global__ void
   inject_kernel(T *__restrict__ coarse, const GridDesc gc, const U *__restrict__ fine,
-                const GridDesc gf)
+                const GridDesc gf, int stride)
   {
     const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (node >= gc.n_nodes)
@@ -428,6 +429,6 @@ namespace ppf
     const int j0 = int(node % gc.nodes[0]);
     const int j1 = int((node / gc.nodes[0]) % gc.nodes[1]);
     const int j2 = dim == 3 ? int(node / (int64_t(gc.nodes[0]) * gc.nodes[1])) : 0;
-    coarse[node] = T(fine[node_index(gf, 2 * j0, 2 * j1, 2 * j2)]);
+    coarse[node] = T(fine[node_index(gf, stride * j0, stride * j1, stride * j2)]);
   }
 } // namespace ppf

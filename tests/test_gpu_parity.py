@@ -11,8 +11,8 @@ import numpy as np
 import pytest
 
 from phasefield_amd import Context, prm
-from phasefield_amd.capi import (FP32, FP64, OP_LHS, OP_LHS_PLAIN, OP_RHS, SCALE_INVM,
-                                 SCALE_INVM_SQRT, SCALE_NONE)
+from phasefield_amd.capi import (AGE_LATEST, FP32, FP64, OP_LHS, OP_LHS_PLAIN, OP_RHS,
+                                 SCALE_INVM, SCALE_INVM_SQRT, SCALE_NONE)
 from oracle.binding import Oracle
 from tests.helpers import (block_to_full, block_to_stored, rel_err, to_full, to_stored)
 
@@ -136,13 +136,13 @@ def test_run_matches_oracle(app, text, steps):
     assert ctx.run(steps) == 0
     assert orc.run(steps) == 0
     for f in range(ctx.info.n_fields):
-        got, ref = ctx.get_field(f).ravel(), orc.get_field(f)
+        got, ref = ctx.get_field(f, AGE_LATEST).ravel(), orc.get_field(f, AGE_LATEST)
         err = rel_err(got, ref)
         print(app, "field", f, err, flush=True)
         assert err <= 1e-10, (f, err)
     # Output diagnostics: l2-norm over deal.II's node set and integral.
     for f in range(ctx.info.n_fields):
-        a, b = ctx.field_norms(f), orc.field_norms(f)
+        a, b = ctx.field_norms(f, AGE_LATEST), orc.field_norms(f, AGE_LATEST)
         assert abs(a[0] - b[0]) <= 1e-10 * max(1.0, abs(b[0]))
         assert abs(a[1] - b[1]) <= 1e-10 * max(1.0, abs(b[1]))
 
@@ -164,14 +164,13 @@ def test_linear_solve_matches_oracle(app, kw):
     ctx.run(0)
     orc.run(0)
     st = ctx.last_solve_stats(1)
-    ref = orc.last_stats if hasattr(orc, "last_stats") else None
     orc2 = Oracle(app, text)
     orc2.solve_block(0)
     ost = orc2.linear_solve(1)
     print(app, kw, "gpu its", st.iterations, "oracle its", ost["iterations"], flush=True)
     assert st.converged and ost["converged"]
     assert abs(st.iterations - ost["iterations"]) <= 1
-    u_gpu, u_ref = ctx.get_field(0).ravel(), orc.get_field(0)
+    u_gpu, u_ref = ctx.get_field(0, AGE_LATEST).ravel(), orc.get_field(0, AGE_LATEST)
     assert rel_err(u_gpu, u_ref) <= 1e-6
 
 
@@ -186,7 +185,7 @@ def test_testbed_gmg_variable_coefficient(device):
     orc2.solve_block(0)
     ost = orc2.linear_solve(1)
     assert st.converged and abs(st.iterations - ost["iterations"]) <= 1
-    assert rel_err(ctx.get_field(1).ravel(), orc.get_field(1)) <= 1e-7
+    assert rel_err(ctx.get_field(1, AGE_LATEST).ravel(), orc.get_field(1, AGE_LATEST)) <= 1e-7
 
 
 def test_mixed_precision_gmg_reaches_fp64_tolerance(device):
@@ -197,7 +196,7 @@ def test_mixed_precision_gmg_reaches_fp64_tolerance(device):
     orc = Oracle("poisson_3d_q2", prm.poisson(dim=3, refinement=3, mg_depth=3, tolerance=1e-9))
     orc.run(0)
     assert st.converged and st.final_residual <= st.tolerance
-    assert rel_err(ctx.get_field(0).ravel(), orc.get_field(0)) <= 1e-6
+    assert rel_err(ctx.get_field(0, AGE_LATEST).ravel(), orc.get_field(0, AGE_LATEST)) <= 1e-6
 
 
 ADJ_TEXT = prm.allen_cahn(dim=3, size=8.0, refinement=3, steps=0, dt=0.01, kappa=1.5)
@@ -245,9 +244,9 @@ def test_explicit_matches_adjoint_forward(device):
     text = prm.allen_cahn(dim=3, size=8.0, refinement=3, steps=4, dt=0.01, L=1.0, kappa=1.5)
     ctx = Context("allen_cahn_3d_q1", text)
     ctx.run(0)
-    phi0 = to_stored(ctx.get_field(0).ravel(), ctx.info)
+    phi0 = to_stored(ctx.get_field(0, AGE_LATEST).ravel(), ctx.info)
     ctx.run(4)
-    phi4 = to_stored(ctx.get_field(0).ravel(), ctx.info)
+    phi4 = to_stored(ctx.get_field(0, AGE_LATEST).ravel(), ctx.info)
     d0, d4 = dev(phi0), torch.zeros(ctx.info.n_nodes, dtype=torch.float64, device="cuda")
     ctx.adjoint_forward(d0.data_ptr(), 1.0, 1.5, 4, FP64, d4.data_ptr())
     assert rel_err(d4.cpu().numpy(), phi4) <= 1e-12
