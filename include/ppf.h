@@ -96,8 +96,11 @@ int ppf_get_info(ppf_ctx *ctx, ppf_info *info);
 /* Copies the name of field `field` into buf (NUL-terminated, at most len bytes). */
 int ppf_field_name(ppf_ctx *ctx, int field, char *buf, int len);
 
-/* Options: "operator_path" = auto|generic|split; "synchronize" = 1|0 (0 leaves
- * ppf_apply_operator asynchronous on the context's stream);
+/* Options: "operator_path" = auto|generic|split|split_node (split: tiled stencil
+ * kernel, split_node: one thread per node); "stencil_rows" = 0|2|4 and
+ * "stencil_z_chunk" = <planes> (0: automatic) tune the tiled kernel's workgroup
+ * shape; "synchronize" = 1|0 (0 leaves ppf_apply_operator asynchronous on the
+ * context's stream);
  * "mg_level_precision" = double|single, "mg_coarse_solver" = smoother|cg,
  * "linear_tolerance" = <value> and "linear_max_iterations" = <n> (applied to every
  * linear block, overriding the parameter file). */

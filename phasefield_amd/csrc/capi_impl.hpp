@@ -216,9 +216,18 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
           c.rt.path = ppf::OperatorPath::Generic;
         else if (v == "split")
           c.rt.path = ppf::OperatorPath::Split;
+        else if (v == "split_node")
+          c.rt.path = ppf::OperatorPath::SplitNode;
         else
-          throw ppf::Error(ppf::status_invalid, "operator_path must be auto|generic|split");
+          throw ppf::Error(ppf::status_invalid,
+                           "operator_path must be auto|generic|split|split_node");
       }
+    else if (k == "stencil_rows")
+      c.rt.stencil_rows = std::stoi(v);
+    else if (k == "stencil_z_chunk")
+      c.rt.stencil_z_chunk = std::stoi(v);
+    else if (k == "stencil_target_blocks")
+      c.rt.stencil_target_blocks = std::max(1, std::stoi(v));
     else if (k == "synchronize")
       c.synchronize = v != "0";
     else if (k == "mg_level_precision")

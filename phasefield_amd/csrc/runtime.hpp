@@ -14,6 +14,7 @@
 #include "grid.hpp"
 #include "operator_kernels.hpp"
 #include "operator_plan.hpp"
+#include "stencil_kernels.hpp"
 #include "vector_kernels.hpp"
 
 #include <prismspf/core/field_attributes.h>
@@ -39,9 +40,13 @@ namespace ppf
 {
   enum class OperatorPath : int
   {
-    Auto    = 0,
-    Generic = 1,
-    Split   = 2
+    Auto      = 0,
+    Generic   = 1,
+    /// Split operator on the tiled stencil kernel (per-node kernel beyond two
+    /// stencil operands).
+    Split     = 2,
+    /// Split operator on the per-node kernel.
+    SplitNode = 3
   };
 
   /// Statistics of the last linear solve of a block.
@@ -539,7 +544,7 @@ namespace ppf
       out.dirichlet_write = dwrite;
 
       const SplitPlan &plan = split_plan<number>(b, lhs);
-      const bool use_split  = path == OperatorPath::Split ||
+      const bool use_split  = path == OperatorPath::Split || path == OperatorPath::SplitNode ||
                              (path == OperatorPath::Auto && plan.eligible);
       if (use_split && !plan.eligible)
         throw Error(status_unsupported, "split path requested but the hook does not "
@@ -548,6 +553,9 @@ namespace ppf
       const unsigned int nb  = grid_for(g.n_nodes, block_size);
       dispatch_block_id(typename App::BlockIds {}, blk.id, [&](auto bid) {
         constexpr int id = decltype(bid)::value;
+        if (use_split && path != OperatorPath::SplitNode &&
+            launch_stencil<number, id>(lhs, g, deps, out, plan))
+          return;
         if (use_split)
           {
             if (lhs)
@@ -577,6 +585,181 @@ namespace ppf
           }
       });
       PPF_HIP_CHECK(hipGetLastError());
+    }
+
+    /// Launches the tiled stencil kernel for a split operator; false if the operator
+    /// has more stencil operands than the kernel stages (the per-node kernel runs it).
+    template <typename number, int id>
+    bool
+    launch_stencil(bool lhs, const GridDesc &g, const DepTable<number> &deps,
+                   const OutTable<number> &out, const SplitPlan &plan)
+    {
+      StencilArgs<number> a {};
+      // Distinct (field, slot) operands of the gradient terms.
+      int op_field[stencil_max_ops], op_slot[stencil_max_ops];
+      a.n_ops = 0;
+      for (int t = 0; t < plan.terms.n_terms; ++t)
+        {
+          const SplitTerm &tm = plan.terms.term[t];
+          int              k  = 0;
+          while (k < a.n_ops && !(op_field[k] == tm.in_field && op_slot[k] == tm.in_slot))
+            ++k;
+          if (k == a.n_ops)
+            {
+              if (a.n_ops == stencil_max_ops)
+                return false;
+              op_field[k] = tm.in_field;
+              op_slot[k]  = tm.in_slot;
+              a.op[k]     = deps.ptr[tm.in_field][tm.in_slot];
+              a.op_dmask[k] =
+                (deps.flags[tm.in_field][tm.in_slot] & slot_constrained)
+                  ? deps.dirichlet_mask[tm.in_field]
+                  : 0u;
+              if (!a.op[k])
+                throw Error(status_internal, "stencil operand without storage");
+              ++a.n_ops;
+            }
+          a.beta[tm.out_field][k] += number(tm.beta);
+        }
+      // Unused operand and value entries point at an array of the lattice size so the
+      // kernel loads them without a branch (their values are never used).
+      const number *any_array = nullptr;
+      for (int f = 0; f < PPF_MAX_FIELDS && !any_array; ++f)
+        if (out.out_mask & (1u << f))
+          any_array = out.dst[f];
+      if (!any_array)
+        throw Error(status_internal, "operator without an output field");
+      for (int k = a.n_ops; k < stencil_max_ops; ++k)
+        a.op[k] = any_array;
+      for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+        for (int s = 0; s < PPF_N_SLOTS; ++s)
+          {
+            a.val_alias[f][s] = -1;
+            a.val[f][s]       = any_array;
+            if (!(deps.flags[f][s] & slot_values))
+              continue;
+            a.val[f][s]       = deps.ptr[f][s];
+            a.val_dmask[f][s] = (deps.flags[f][s] & slot_constrained) ? deps.dirichlet_mask[f] : 0u;
+            for (int k = 0; k < a.n_ops; ++k)
+              if (op_field[k] == f && op_slot[k] == s)
+                a.val_alias[f][s] = static_cast<signed char>(k);
+          }
+
+      double volume = 1.0;
+      for (int d = 0; d < 3; ++d)
+        {
+          a.n[d]        = d < dim ? g.nodes[d] : 1;
+          a.periodic[d] = d < dim ? g.periodic[d] : 0;
+          if (a.periodic[d] && a.n[d] < 2)
+            return false;
+          // Interior taps (centre, +-1, +-2) in units of 1/h^2 per coordinate parity;
+          // see stencil_kernels.hpp for their derivation from the GLL stiffness.
+          const double tq2[2][3] = {{14.0, -8.0, 1.0}, {8.0, -4.0, 0.0}};
+          const double tq1[3]    = {2.0, -1.0, 0.0};
+          for (int par = 0; par < 2; ++par)
+            for (int k = 0; k < 3; ++k)
+              {
+                const double t = degree == 2 ? tq2[par][k] : tq1[k];
+                a.tap[d][par][k] = d < dim ? number(t / (g.h[d] * g.h[d])) : number(0);
+              }
+          for (int c = 0; c < 3; ++c)
+            {
+              const double m = d < dim ? class_mass_units<degree>(c) * g.h[d] : 1.0;
+              a.sfac[d][c]   = number(out.scaling == scale_none ? m
+                                      : out.scaling == scale_invm ? 1.0 : std::sqrt(m));
+            }
+          if (d < dim)
+            {
+              a.q_lower[d] = g.lower[d];
+              a.q_step[d]  = g.h[d] / degree;
+              volume *= g.h[d];
+            }
+        }
+      a.element_volume = volume;
+      a.any_dmask      = 0;
+      for (int k = 0; k < a.n_ops; ++k)
+        a.any_dmask |= a.op_dmask[k];
+      for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+        {
+          for (int s = 0; s < PPF_N_SLOTS; ++s)
+            a.any_dmask |= a.val_dmask[f][s];
+          if (out.out_mask & (1u << f))
+            a.any_dmask |= out.dirichlet_mask[f];
+        }
+      if (int64_t(a.n[0]) * a.n[1] >= (int64_t(1) << 31))
+        return false;
+
+      int rows = stencil_rows;
+      if (rows == 0)
+        rows = g.n_nodes >= (int64_t(8) << 20) ? 4 : 2;
+      if (rows != 2 && rows != 4)
+        throw Error(status_invalid, "stencil_rows must be 2 or 4");
+      if (a.n_ops > 1)
+        rows = 2;
+      const int ty = stencil_waves * rows;
+      a.tiles_x    = (a.n[0] + stencil_tx - 1) / stencil_tx;
+      a.tiles_y    = (a.n[1] + ty - 1) / ty;
+      if constexpr (dim == 3)
+        {
+          int zc = stencil_z_chunk;
+          if (zc <= 0)
+            {
+              const int64_t xy = int64_t(a.tiles_x) * a.tiles_y;
+              zc = int((int64_t(a.n[2]) * xy + stencil_target_blocks - 1) / stencil_target_blocks);
+              zc = std::max(zc, 4 * degree);
+            }
+          a.z_chunk  = std::min(zc, a.n[2]);
+          a.chunks_z = (a.n[2] + a.z_chunk - 1) / a.z_chunk;
+        }
+      else
+        {
+          a.z_chunk  = 1;
+          a.chunks_z = 1;
+        }
+      const int64_t nblocks = int64_t(a.tiles_x) * a.tiles_y * a.chunks_z;
+      if (nblocks >= (int64_t(1) << 31))
+        return false;
+      a.n_blocks = int(nblocks);
+
+      const auto &pde = *pde_of<number>();
+      const dim3  grid(a.n_blocks), threads(stencil_tx, stencil_waves);
+      auto        go = [&](auto rows_c, auto ops_c) {
+        constexpr int RR = decltype(rows_c)::value;
+        constexpr int NS = decltype(ops_c)::value;
+        if (lhs)
+          hipLaunchKernelGGL(
+            (stencil_operator_kernel<PDE<number>, dim, degree, number, true, id, App::n_fields, RR, NS>),
+            grid, threads, 0, stream, pde, a, out, timer);
+        else
+          hipLaunchKernelGGL(
+            (stencil_operator_kernel<PDE<number>, dim, degree, number, false, id, App::n_fields, RR, NS>),
+            grid, threads, 0, stream, pde, a, out, timer);
+      };
+      using I2 = std::integral_constant<int, 2>;
+      using I4 = std::integral_constant<int, 4>;
+      using S1 = std::integral_constant<int, 1>;
+      using S2 = std::integral_constant<int, 2>;
+      // Two staged operands use 2 rows per wave (4 would spill registers in FP64).
+      if (a.n_ops > 1)
+        go(I2 {}, S2 {});
+      else if (rows == 2)
+        go(I2 {}, S1 {});
+      else
+        go(I4 {}, S1 {});
+      if (out.dirichlet_write == dirichlet_value)
+        for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+          if ((out.out_mask & (1u << f)) && out.dirichlet_mask[f])
+            {
+              int64_t face = 0;
+              for (int d = 0; d < 3; ++d)
+                face = std::max(face, int64_t(a.n[(d + 1) % 3]) * a.n[(d + 2) % 3]);
+              hipLaunchKernelGGL(dirichlet_faces_kernel<number>,
+                                 dim3(unsigned((face + 255) / 256), 6), dim3(256), 0, stream,
+                                 out.dst[f], out.dirichlet_values[f], a.n[0], a.n[1], a.n[2],
+                                 a.periodic[0], a.periodic[1], a.periodic[2],
+                                 out.dirichlet_mask[f]);
+            }
+      return true;
     }
 
     /// Operator diagonal of the lhs of block `b` for block field k, scaled like the
@@ -1486,6 +1669,11 @@ namespace ppf
     GridDesc                               fine {};
     hipStream_t                            stream = nullptr;
     OperatorPath                           path   = OperatorPath::Auto;
+    /// Tile rows per wave of the stencil kernel (0: chosen from the lattice size) and
+    /// z-planes per workgroup (0: chosen to give about `stencil_target_blocks` groups).
+    int                                    stencil_rows    = 0;
+    int                                    stencil_z_chunk = 0;
+    int                                    stencil_target_blocks = 2048;
 
     std::vector<std::vector<DeviceBuffer<double>>> storage;
     std::vector<DeviceBuffer<double>>              dirichlet_values;

@@ -1,0 +1,468 @@
+// Tiled realization of the collocation-split operator (see operator_kernels.hpp).
+//
+// With GLL collocation the assembled operator of a split hook is, at lattice node i,
+//
+//   (A u)_i = M_i * ( r(values at i) + sum_terms beta * (L u)_i ),
+//   (L u)_i = sum_d sum_o W_d(i_d, o) u_{i + o e_d},
+//
+// with M_i = prod_d m_d(i_d) the lumped mass, r the hook's value term and W_d the 1-D
+// taps (assembled 1-D stiffness row) / (assembled 1-D lumped mass) along axis d.  This
+// follows from the tensor-product stiffness K = sum_d (prod_{e!=d} M_e) (x) K_d: the
+// transverse masses are M_i / m_d(i_d).  The reference computes the same operator cell
+// by cell (src/solvers/mf_operator.cc:17-80, FEEvaluation with GLL collocation,
+// include/prismspf/core/system_wide.h:40-61); only the order of the floating-point sums
+// differs.  Output scaling M^-1 (explicit update) cancels the mass: r + beta (L u).
+//
+// Interior taps, in units of 1/h^2 (K and w of gll_basis.h):
+//   Q1: K = [[1,-1],[-1,1]], w = (1/2, 1/2): (centre, +-1) = (2, -1);
+//   Q2: K = [[7,-8,1],[-8,16,-8],[1,-8,7]]/3, w = (1/6, 2/3, 1/6): odd node
+//       (8, -4, 0), vertex shared by two cells (14, -8, 1) for (centre, +-1, +-2).
+// At a non-periodic boundary the row has only one cell; with mirror ghosts
+// u[-k] = u[k], u[n-1+k] = u[n-1-k] the interior taps give exactly that one-sided row
+// (Q2: (1,-8,14,-8,1) -> (14,-16,2); Q1: (-1,2,-1) -> (2,-2)), so taps depend only on
+// the parity of the coordinate and boundaries are handled by the source index.
+//
+// Work decomposition (3-D): a workgroup of 4 wavefronts owns a tile of 64 x (4R) nodes
+// in the x-y plane and marches over a chunk of z-planes.  Lane = x; wave w owns rows
+// w, w+4, ..., w+4(R-1) (wave-uniform y).
+//   * Every stencil operand value, halo value and hook value input is loaded
+//     D = p+1 planes before the plane that consumes it, into a register ring whose
+//     slot indices are compile-time constants (the z-loop is unrolled by the ring
+//     length), so no load is waited for in the step that issues it.
+//   * The centre plane of each operand plus a p-wide halo is staged in LDS
+//     (double-buffered: one barrier per plane) for the x and y taps; the z taps read
+//     the ring.
+//   * The hook runs inlined at every node and each output is written once.
+//   * Tiles whose read range touches no Dirichlet face run a step variant without
+//     any face test.
+//   * Consecutive tiles are mapped to one XCD (workgroups are dealt round-robin to the
+//     8 XCDs), so halo rows and chunk boundaries are re-read from the same L2.
+// 2-D lattices are the single-plane case (no z ring).
+#pragma once
+
+#include "operator_kernels.hpp"
+
+#include <type_traits>
+#include <utility>
+
+namespace ppf
+{
+  constexpr int stencil_tx      = 64; // tile width in x: one wavefront
+  constexpr int stencil_waves   = 4;  // wavefronts per workgroup
+  constexpr int stencil_threads = stencil_tx * stencil_waves;
+  constexpr int stencil_max_ops = 2;
+
+  template <typename number>
+  struct StencilArgs
+  {
+    int    n[3];        // nodes per axis (n[2] = 1 for 2-D lattices)
+    int    periodic[3];
+    int    tiles_x, tiles_y, chunks_z, z_chunk, n_blocks;
+    /// Taps (centre, +-1, +-2) per axis and coordinate parity (index: i & 1 for Q2,
+    /// both equal for Q1), already divided by h^2.
+    number tap[3][2][3];
+    /// Output scale factor per axis and node class (0 odd / 1 even or Q1 interior /
+    /// 2 non-periodic boundary): the 1-D lumped mass for unscaled output, 1 for M^-1
+    /// scaling, its square root for M^-1/2 scaling.
+    number sfac[3][3];
+    double q_lower[3], q_step[3];
+    double element_volume;
+    int    n_ops;
+    const number *op[stencil_max_ops];
+    unsigned int  op_dmask[stencil_max_ops]; // Dirichlet faces read as zero (0: plain)
+    /// Summed coefficient of operand k in output field f.
+    number        beta[PPF_MAX_FIELDS][stencil_max_ops];
+    const number *val[PPF_MAX_FIELDS][PPF_N_SLOTS]; // value inputs of the hook
+    unsigned int  val_dmask[PPF_MAX_FIELDS][PPF_N_SLOTS];
+    /// Operand index k if the input is operand k (read from the ring), -1 otherwise.
+    /// Every `op` and `val` pointer is valid (the host points unused entries at an
+    /// array of the lattice size), so loads need no branch.
+    signed char   val_alias[PPF_MAX_FIELDS][PPF_N_SLOTS];
+    /// Union of all face masks the operator reads or writes (0: no face test at all).
+    unsigned int  any_dmask;
+  };
+
+  /// Source node along one axis for lattice coordinate i, which may lie up to a stencil
+  /// radius outside [0, n): periodic axes wrap, other axes reflect at the boundary node
+  /// (the mirror ghosts above).  Coordinates further out are reached only by lanes and
+  /// rows without output nodes and are clamped.
+  PPF_HD PPF_INLINE int
+  source_coord(int i, int n, int periodic)
+  {
+    if (periodic)
+      i = i < 0 ? i + n : (i >= n ? i - n : i);
+    else
+      i = i < 0 ? -i : (i >= n ? 2 * (n - 1) - i : i);
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+  }
+
+  /// Boundary-face bits of lattice coordinate i on `axis` (bit 2 axis: lower face,
+  /// bit 2 axis + 1: upper face; none in periodic axes).
+  PPF_HD PPF_INLINE unsigned int
+  face_bits(int i, int n, int periodic, int axis)
+  {
+    if (periodic)
+      return 0u;
+    return (i == 0 ? 1u << (2 * axis) : 0u) | (i == n - 1 ? 1u << (2 * axis + 1) : 0u);
+  }
+
+  /// Node class of coordinate i for the output scale factor.
+  template <int degree>
+  PPF_HD PPF_INLINE int
+  scale_class(int i, int n, int periodic)
+  {
+    if (!periodic && (i == 0 || i == n - 1))
+      return 2;
+    return (degree == 2 && (i & 1)) ? 0 : 1;
+  }
+
+  /// Host-side 1-D lumped mass of scale class c, in units of h.
+  template <int degree>
+  inline double
+  class_mass_units(int c)
+  {
+    if (degree == 2)
+      return c == 0 ? 2.0 / 3.0 : (c == 1 ? 1.0 / 3.0 : 1.0 / 6.0);
+    return c == 2 ? 0.5 : 1.0;
+  }
+
+  /// True if the read range [lo - rad, hi + rad] of a tile along an axis contains a
+  /// boundary node (only non-periodic axes have one).
+  PPF_HD PPF_INLINE bool
+  touches_face(int lo, int hi, int rad, int n, int periodic)
+  {
+    return !periodic && (lo - rad <= 0 || hi + rad >= n - 1);
+  }
+
+  template <class PDE, int dim, int degree, typename number, bool lhs, int block_id, int NF,
+            int R, int NS>
+  __global__ void __launch_bounds__(stencil_threads)
+  stencil_operator_kernel(const PDE                       pde,
+                          const StencilArgs<number>       a,
+                          const OutTable<number>          out,
+                          const prismspf::SimulationTimer timer)
+  {
+    constexpr int RAD = degree;
+    constexpr int T   = 2 * RAD + 1;
+    constexpr int TY  = stencil_waves * R;
+    constexpr int LX  = stencil_tx + 2 * RAD;
+    constexpr int LY  = TY + 2 * RAD;
+    constexpr int D   = RAD + 1; // load distance in planes
+    constexpr int S   = T + 1;   // ring length
+    __shared__ number lds[2][NS][LY][LX];
+
+    const int tx  = threadIdx.x;
+    const int w   = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int tid = w * stencil_tx + tx;
+    const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
+    const int p0 = a.periodic[0], p1 = a.periodic[1], p2 = a.periodic[2];
+
+    const int b   = blockIdx.x;
+    const int q8  = a.n_blocks / 8;
+    const int lb  = b < 8 * q8 ? (b % 8) * q8 + b / 8 : b;
+    const int t_x = lb % a.tiles_x;
+    const int t_r = lb / a.tiles_x;
+    const int t_y = t_r % a.tiles_y;
+    const int chk = t_r / a.tiles_y;
+    const int x0  = t_x * stencil_tx;
+    const int y0  = t_y * TY;
+    const int zb  = dim == 3 ? chk * a.z_chunk : 0;
+    const int ze  = dim == 3 ? min(zb + a.z_chunk, n2) : 1;
+    const int64_t plane_size = int64_t(n0) * n1;
+
+    // Own column (lane-constant).
+    const int          x    = x0 + tx;
+    const bool         xout = x < n0; // owns output nodes
+    const int          xs   = source_coord(x, n0, p0);
+    const unsigned int fbx  = face_bits(xs, n0, p0, 0);
+    const int          pxi  = degree == 2 ? (xs & 1) : 0;
+    const number       wx0  = pxi ? a.tap[0][1][0] : a.tap[0][0][0];
+    const number       wx1  = pxi ? a.tap[0][1][1] : a.tap[0][0][1];
+    const number       wx2  = pxi ? a.tap[0][1][2] : a.tap[0][0][2];
+    const int          cx   = scale_class<degree>(xs, n0, p0);
+    const number       sx   = cx == 0 ? a.sfac[0][0] : (cx == 1 ? a.sfac[0][1] : a.sfac[0][2]);
+
+    // Own rows (wave-uniform).
+    int          ys[R];
+    bool         yout[R];
+    unsigned int fby[R];
+    int64_t      own_off[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      {
+        const int y = y0 + w + stencil_waves * r;
+        yout[r]     = y < n1;
+        ys[r]       = source_coord(y, n1, p1);
+        fby[r]      = face_bits(ys[r], n1, p1, 1);
+        own_off[r]  = int64_t(xs) + int64_t(n0) * ys[r];
+      }
+
+    // Halo assignments: wave w < 2 RAD loads one y-halo row of the tile's columns;
+    // threads tid < 2 RAD TY load one x-halo node (the others repeat the y-halo load).
+    const int          hy_row = w < RAD ? w : TY + w; // LDS row
+    const bool         hy_on  = w < 2 * RAD;
+    const int          hy_y   = source_coord(y0 - RAD + hy_row, n1, p1);
+    const unsigned int fb_hy  = fbx | face_bits(hy_y, n1, p1, 1);
+    const int          hx_k   = tid % (2 * RAD);
+    const int          hx_row = tid / (2 * RAD); // tile row
+    const int          hx_col = hx_k < RAD ? hx_k : stencil_tx + hx_k; // LDS column
+    const bool         hx_on  = tid < 2 * RAD * TY;
+    const int          hx_x   = source_coord(x0 - RAD + hx_col, n0, p0);
+    const int          hx_y   = source_coord(y0 + hx_row, n1, p1);
+    const unsigned int fb_hx  = face_bits(hx_x, n0, p0, 0) | face_bits(hx_y, n1, p1, 1);
+    const int64_t      hy_off = int64_t(xs) + int64_t(n0) * hy_y;
+    const int64_t      hx_off = hx_on ? int64_t(hx_x) + int64_t(n0) * hx_y : hy_off;
+
+    // Register rings: slot of absolute plane p = (p - zb + RAD) mod S.
+    number       ring[NS][R][S];
+    number       ring_hy[NS][S], ring_hx[NS][S];
+    number       ring_v[S][R][NF][PPF_N_SLOTS];
+    unsigned int ring_fb[S]; // face bits of the plane in the slot (uniform)
+
+    auto load_own = [&](int z, int slot) {
+      const int     zs = source_coord(z, n2, p2);
+      const int64_t po = int64_t(zs) * plane_size;
+      ring_fb[slot]    = face_bits(zs, n2, p2, 2);
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          ring[s][r][slot] = a.op[s][po + own_off[r]];
+    };
+    auto load_inputs = [&](int z, int slot) {
+      const int64_t po = int64_t(source_coord(z, n2, p2)) * plane_size;
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        {
+          ring_hy[s][slot] = a.op[s][po + hy_off];
+          ring_hx[s][slot] = a.op[s][po + hx_off];
+        }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+          for (int s = 0; s < PPF_N_SLOTS; ++s)
+            ring_v[slot][r][f][s] = a.val[f][s][po + own_off[r]];
+    };
+
+    // One plane.  MASKED: face tests (constrained reads, Dirichlet outputs); without
+    // it the tile's read range contains no boundary node of a masked face.
+    auto step = [&]<int PH, bool MASKED>(std::integral_constant<int, PH>,
+                                          std::integral_constant<bool, MASKED>, int z) {
+      constexpr int cs  = (PH + RAD) % S; // centre slot
+      constexpr int ns  = (PH + S - 1) % S;
+      constexpr int buf = PH & 1;
+      if constexpr (dim == 3)
+        {
+          load_own(min(z + D, ze - 1 + RAD), ns);
+          load_inputs(min(z + D, ze - 1), ns);
+        }
+
+      // Operand value of ring slot k, row r, zeroed on its Dirichlet faces in masked
+      // tiles.  (Entries of absent operands hold arbitrary values; their sums are
+      // discarded below.)
+      auto opv = [&](int s, int r, int k) {
+        if constexpr (MASKED)
+          return ((fbx | fby[r] | ring_fb[k]) & a.op_dmask[s]) ? number(0) : ring[s][r][k];
+        else
+          return ring[s][r][k];
+      };
+
+      // Stage the centre plane.
+      number ctr[NS][R];
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        {
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            {
+              ctr[s][r] = opv(s, r, cs);
+              lds[buf][s][RAD + w + stencil_waves * r][RAD + tx] = ctr[s][r];
+            }
+          number hyv = ring_hy[s][cs], hxv = ring_hx[s][cs];
+          if constexpr (MASKED)
+            {
+              const unsigned int dm = a.op_dmask[s];
+              hyv = ((fb_hy | ring_fb[cs]) & dm) ? number(0) : hyv;
+              hxv = ((fb_hx | ring_fb[cs]) & dm) ? number(0) : hxv;
+            }
+          if (hy_on)
+            lds[buf][s][hy_row][RAD + tx] = hyv;
+          if (hx_on)
+            lds[buf][s][RAD + hx_row][hx_col] = hxv;
+        }
+
+      // z taps and scale factor (uniform).
+      const unsigned int fbz  = ring_fb[cs];
+      const int64_t      zoff = int64_t(z) * plane_size;
+      number             wz0 = number(0), wz1 = number(0), wz2 = number(0), sz = number(1);
+      if constexpr (dim == 3)
+        {
+          const bool pz = degree == 2 && (z & 1);
+          wz0           = pz ? a.tap[2][1][0] : a.tap[2][0][0];
+          wz1           = pz ? a.tap[2][1][1] : a.tap[2][0][1];
+          wz2           = pz ? a.tap[2][1][2] : a.tap[2][0][2];
+          const int c  = scale_class<degree>(z, n2, p2);
+          sz           = c == 0 ? a.sfac[2][0] : (c == 1 ? a.sfac[2][1] : a.sfac[2][2]);
+        }
+      __syncthreads();
+
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        {
+          const bool   py  = degree == 2 && (ys[r] & 1);
+          const int    cy  = scale_class<degree>(ys[r], n1, p1);
+          const number wy1 = py ? a.tap[1][1][1] : a.tap[1][0][1];
+          const number wy2 = py ? a.tap[1][1][2] : a.tap[1][0][2];
+          const number c0  = wx0 + ((py ? a.tap[1][1][0] : a.tap[1][0][0]) + wz0);
+          const number syz =
+            (cy == 0 ? a.sfac[1][0] : (cy == 1 ? a.sfac[1][1] : a.sfac[1][2])) * sz;
+          const int ly = RAD + w + stencil_waves * r;
+          const int lx = RAD + tx;
+
+          number lap[NS];
+#pragma unroll
+          for (int s = 0; s < NS; ++s)
+            {
+              const number(&L)[LY][LX] = lds[buf][s];
+              number acc = c0 * ctr[s][r];
+              acc += wx1 * (L[ly][lx - 1] + L[ly][lx + 1]);
+              acc += wy1 * (L[ly - 1][lx] + L[ly + 1][lx]);
+              if constexpr (dim == 3)
+                acc += wz1 * (opv(s, r, (cs + S - 1) % S) + opv(s, r, (cs + 1) % S));
+              if constexpr (RAD == 2)
+                {
+                  acc += wx2 * (L[ly][lx - 2] + L[ly][lx + 2]);
+                  acc += wy2 * (L[ly - 2][lx] + L[ly + 2][lx]);
+                  if constexpr (dim == 3)
+                    acc += wz2 * (opv(s, r, (cs + S - 2) % S) + opv(s, r, (cs + 2) % S));
+                }
+              lap[s] = s < a.n_ops ? acc : number(0);
+            }
+
+          // Hook at the node; lanes and rows without an output node evaluate it on
+          // clamped values and store nothing.
+          const unsigned int fb    = fbx | fby[r] | fbz;
+          const int64_t      node  = zoff + own_off[r];
+          prismspf::FieldContainer<dim, degree, number> fc;
+          fc.q_point[0] = number(a.q_lower[0] + xs * a.q_step[0]);
+          fc.q_point[1] = number(a.q_lower[1] + ys[r] * a.q_step[1]);
+          if constexpr (dim == 3)
+            fc.q_point[2] = number(a.q_lower[2] + z * a.q_step[2]);
+          fc.element_volume = number(a.element_volume);
+#pragma unroll
+          for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+#pragma unroll
+            for (int s = 0; s < PPF_N_SLOTS; ++s)
+              {
+                number v = number(0);
+                if (f < NF)
+                  {
+                    const int fi = f < NF ? f : 0;
+                    const int al = a.val_alias[fi][s];
+                    v            = ring_v[cs][r][fi][s];
+                    if constexpr (MASKED)
+                      v = (fb & a.val_dmask[fi][s]) ? number(0) : v;
+                    if (al == 0)
+                      v = ctr[0][r];
+                    if (NS > 1 && al == 1)
+                      v = ctr[NS > 1 ? 1 : 0][r];
+                  }
+                fc.value[f][s] = v;
+#pragma unroll
+                for (int d = 0; d < dim; ++d)
+                  fc.gradient[f][s][d] = number(0);
+              }
+          fc.clear_terms();
+          call_hook<PDE, dim, degree, number, lhs, block_id>(pde, fc, timer);
+
+          const number scale = sx * syz;
+          const bool   valid = xout && yout[r];
+#pragma unroll
+          for (int f = 0; f < NF; ++f)
+            {
+              if (!(out.out_mask & (1u << f)))
+                continue;
+              number acc = (fc.value_term_mask & (1u << f)) ? fc.value_term[f] : number(0);
+#pragma unroll
+              for (int k = 0; k < NS; ++k)
+                acc += a.beta[f][k] * lap[k];
+              if constexpr (MASKED)
+                {
+                  // Dirichlet nodes: zero (operator rows dropped), or left to
+                  // dirichlet_faces_kernel, which writes the constraint values.
+                  const bool dir = (fb & out.dirichlet_mask[f]) != 0;
+                  if (valid && !(dir && out.dirichlet_write == dirichlet_value))
+                    out.dst[f][node] = dir ? number(0) : scale * acc;
+                }
+              else if (valid)
+                out.dst[f][node] = scale * acc;
+            }
+        }
+    };
+
+    auto run = [&]<bool MASKED>(std::integral_constant<bool, MASKED> m) {
+      if constexpr (dim == 3)
+        {
+          // Prologue: own planes zb-RAD .. zb+RAD (slots 0 .. 2 RAD), halo and value
+          // inputs of planes zb .. zb+RAD (slots RAD .. 2 RAD).
+#pragma unroll
+          for (int k = 0; k < T; ++k)
+            load_own(min(zb - RAD + k, ze - 1 + RAD), k);
+#pragma unroll
+          for (int k = RAD; k < T; ++k)
+            load_inputs(min(zb - RAD + k, ze - 1), k);
+          for (int z = zb; z < ze; z += S)
+            {
+              [&]<int... PH>(std::integer_sequence<int, PH...>) {
+                ((z + PH < ze ? step(std::integral_constant<int, PH> {}, m, z + PH) : void()),
+                 ...);
+              }(std::make_integer_sequence<int, S> {});
+            }
+        }
+      else
+        {
+          load_own(0, RAD);
+          load_inputs(0, RAD);
+          step(std::integral_constant<int, 0> {}, m, 0);
+        }
+    };
+
+    const bool masked = a.any_dmask != 0 &&
+                        (touches_face(x0, x0 + stencil_tx - 1, RAD, n0, p0) ||
+                         touches_face(y0, y0 + TY - 1, RAD, n1, p1) ||
+                         (dim == 3 && touches_face(zb, ze - 1, RAD, n2, p2)));
+    if (masked)
+      run(std::integral_constant<bool, true> {});
+    else
+      run(std::integral_constant<bool, false> {});
+  }
+
+  /// Writes the constraint values of the Dirichlet faces in `mask` into dst (explicit
+  /// update semantics, reference explicit_solver.h:63-77 with constraints distributed).
+  /// One thread per node of a face; blockIdx.y is the face (2 axis + upper).
+  template <typename number>
+  __global__ void __launch_bounds__(256)
+  dirichlet_faces_kernel(number *__restrict__ dst, const number *__restrict__ values, int n0,
+                         int n1, int n2, int p0, int p1, int p2, unsigned int mask)
+  {
+    const int face   = blockIdx.y;
+    const int axis   = face >> 1;
+    const int n[3]   = {n0, n1, n2};
+    const int per[3] = {p0, p1, p2};
+    if (!(mask & (1u << face)) || per[axis] || n[axis] < 1)
+      return;
+    const int     u = axis == 0 ? 1 : 0;
+    const int     v = axis == 2 ? 1 : 2;
+    const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= int64_t(n[u]) * n[v])
+      return;
+    int c[3];
+    c[axis]           = (face & 1) ? n[axis] - 1 : 0;
+    c[u]              = int(t % n[u]);
+    c[v]              = int(t / n[u]);
+    const int64_t idx = int64_t(c[0]) + int64_t(n0) * (int64_t(c[1]) + int64_t(n1) * c[2]);
+    dst[idx]          = values[idx];
+  }
+} // namespace ppf
