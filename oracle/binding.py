@@ -61,6 +61,8 @@ class Oracle:
             lib.oracle_adjoint_vjp.argtypes = [
                 c_void_p, _dp, _dp, c_double, c_double, c_uint, _dp, _dp, _dp,
             ]
+            lib.oracle_mg_level_ndofs.argtypes = [c_void_p, c_int, c_int, POINTER(c_longlong)]
+            lib.oracle_mg_transfer.argtypes = [c_void_p, c_int, c_int, c_int, _dp, _dp]
             Oracle._libs[app] = lib
         self.lib = Oracle._libs[app]
         if threads:
@@ -163,3 +165,22 @@ class Oracle:
         self._check(self.lib.oracle_adjoint_vjp(self.h, _ptr(p), _ptr(lm), L, kappa, n,
                                                 ctypes.byref(dl), ctypes.byref(dk), _ptr(out)))
         return dl.value, dk.value, out
+
+    def mg_level_ndofs(self, block_id, level):
+        n = c_longlong()
+        self._check(self.lib.oracle_mg_level_ndofs(self.h, block_id, level, ctypes.byref(n)))
+        return n.value
+
+    def mg_prolongate(self, block_id, level, coarse):
+        """fine = P coarse between V-cycle levels level-1 -> level (coarsest first)."""
+        c = np.ascontiguousarray(coarse, dtype=np.float64)
+        out = np.empty(self.mg_level_ndofs(block_id, level))
+        self._check(self.lib.oracle_mg_transfer(self.h, block_id, level, 0, _ptr(c), _ptr(out)))
+        return out
+
+    def mg_restrict(self, block_id, level, fine):
+        """coarse = P^T fine between V-cycle levels level -> level-1."""
+        f = np.ascontiguousarray(fine, dtype=np.float64)
+        out = np.empty(self.mg_level_ndofs(block_id, level - 1))
+        self._check(self.lib.oracle_mg_transfer(self.h, block_id, level, 1, _ptr(f), _ptr(out)))
+        return out

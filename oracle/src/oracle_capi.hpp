@@ -282,4 +282,42 @@ oracle_adjoint_vjp(void *h, const double *phi0, const double *lam, double L, dou
   });
 }
 
+/// Number of DoFs (full lattice) of multigrid level `level` of block `block_id`
+/// (levels of the block's V-cycle, coarsest first).
+int
+oracle_mg_level_ndofs(void *h, int block_id, int level, long long *ndofs)
+{
+  return oracle::guard(h, [&](auto &m) {
+    m.setup_mg(m.block_index(block_id));
+    if (level < 0 || level >= int(m.mg.size()))
+      throw std::invalid_argument("level out of range");
+    *ndofs = m.mg[level].lvl.mesh.ndofs;
+  });
+}
+
+/// Transfer of field position 0 of block `block_id` between level `level` (fine) and
+/// level - 1 (coarse): direction 0 prolongates (out_fine = P in_coarse), direction 1
+/// restricts (out_coarse = P^T in_fine).
+int
+oracle_mg_transfer(void *h, int block_id, int level, int direction, const double *in,
+                   double *out)
+{
+  return oracle::guard(h, [&](auto &m) {
+    const int b = m.block_index(block_id);
+    m.setup_mg(b);
+    if (level < 1 || level >= int(m.mg.size()))
+      throw std::invalid_argument("level out of range");
+    const auto &F     = m.mg[level].lvl;
+    const auto &C     = m.mg[level - 1].lvl;
+    const int   field = m.block_fields(b)[0];
+    if (direction == 0)
+      {
+        std::fill(out, out + F.mesh.ndofs, 0.0);
+        m.prolongate_add(F, C, 0, field, in, out);
+      }
+    else
+      m.restrict_to(F, C, field, in, out);
+  });
+}
+
 } // extern "C"

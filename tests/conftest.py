@@ -20,3 +20,14 @@ def device():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="session")
+def oracle_libs():
+    """Builds the CPU oracle libraries that are missing (about 10 s for all)."""
+    from oracle import build
+
+    missing = [a for a in build.APPS if not build.library_path(a).exists()]
+    if missing:
+        build.build(missing)
+    return True

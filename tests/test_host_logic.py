@@ -1,0 +1,90 @@
+"""Host-side logic on the CPU: the shared C++ headers (dependency strings, solve-block
+rules, parameter files) and the C-ABI libraries' exported symbols (no GPU calls)."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+@pytest.fixture(scope="module")
+def host_logic(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("host") / "host_logic"
+    subprocess.run(["g++", "-std=c++20", "-O1", f"-I{ROOT / 'phasefield_amd' / 'include'}",
+                    str(ROOT / "tests" / "native" / "host_logic.cc"), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    return dict(line.split(": ", 1) for line in out.splitlines())
+
+
+def test_dependency_strings(host_logic):
+    # EvaluationFlags: values 1, gradients 2, hessians 4 (reference dependencies.h:30-213)
+    assert host_logic["c.flag"] == "5"          # "c" + "hess(c)"
+    assert host_logic["c.old1"] == "2"          # "grad(old_1(c))"
+    assert host_logic["eta.old_size"] == "3"    # "old_3(eta)" extends the age list
+    assert host_logic["eta.old3"] == "1"
+    assert host_logic["v.src"] == "3"           # "lhs(v)" + "grad(lhs(v))"
+    assert host_logic["eta.src"] == "2"         # "grad(change(eta))"
+    assert host_logic["unknown"].endswith("not recognized: grad(q)")
+
+
+def test_solve_block_rules(host_logic):
+    assert "current value of a field should never appear in the RHS" in host_logic["own_current"]
+    assert "explicit solves should have no LHS dependencies" in host_logic["explicit_lhs"]
+    assert "src_flag" in host_logic["linear_no_src"]
+    assert host_logic["linear_ok"] == "[]"
+
+
+def test_parameter_file(host_logic):
+    assert host_logic["size"] == "12.5 3"
+    assert host_logic["periodic"] == "1 0"
+    assert host_logic["refinement"] == "3"
+    assert host_logic["n_cells"] == "16 8"          # subdivisions * 2^refinement
+    assert host_logic["dt"] == "0.25"
+    assert host_logic["n_increments"] == "10"
+    assert host_logic["outputs"] == "0 3 6 9"       # EQUAL_SPACING, period max(1, 10/3)
+    assert host_logic["D0"] == "0.7" and host_logic["n"] == "4"
+    assert "not of type INT" in host_logic["const_type"]
+    assert host_logic["bc_c"] == "0=1 1=0 2=0 3=1"  # Dirichlet=1, Natural=0 per face
+    # RMSEPerField=1, tolerance, iterations, GMG=2, depth
+    assert host_logic["lin"] == "1 1e-09 77 2 3"
+    assert "Invalid preconditioner type: ILU" in host_logic["bad_precond"]
+    assert "Missing 'end'" in host_logic["unterminated"]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "ppf.h").read_text()
+    return sorted(set(re.findall(r"\b(ppf_[a-z_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    from phasefield_amd.capi import EXPORTED_SYMBOLS
+
+    assert declared_symbols() == sorted(EXPORTED_SYMBOLS)
+
+
+@pytest.fixture(scope="module")
+def product_libs():
+    from phasefield_amd import build
+
+    missing = [a for a in build.APPS if not build.library_path(a).exists()]
+    if missing:
+        build.build(missing)
+    return [build.library_path(a) for a in build.APPS]
+
+
+def test_libraries_export_every_declared_symbol(product_libs):
+    """Each application library loads without a GPU and exports the full C-ABI."""
+    for path in product_libs:
+        lib = ctypes.CDLL(str(path))
+        missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+        assert not missing, f"{path.name} lacks {missing}"
+        lib.ppf_app_name.restype = ctypes.c_char_p
+        app = path.name[len("libppf_"):-len(".so")]
+        assert lib.ppf_app_name().decode() == app
