@@ -28,6 +28,7 @@ import json
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -44,6 +45,9 @@ def parse():
                    help="target duration of the CPU-baseline sample (0 disables it)")
     p.add_argument("--no-extra", action="store_true", help="skip the Poisson measurements")
     p.add_argument("--kernel-reps", type=int, default=50)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="also run a plain device copy of 16,974,593 doubles (counter "
+                        "calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
     return p.parse_args()
 
 
@@ -54,8 +58,11 @@ def cpu_cores() -> int:
         return os.cpu_count() or 1
 
 
-def time_kernel(ctx, torch, block_id, which, scaling, d_src, d_dst, reps):
-    """Average duration of one operator launch, from HIP events on the library's stream."""
+def time_kernel(ctx, torch, block_id, which, scaling, d_src, d_dst, reps, flush=None):
+    """Average duration of one operator launch, from HIP events on the library's stream.
+
+    flush: optional device tensor written between launches (evicts L2 and the Infinity
+    Cache); the launches are then timed one by one."""
     from phasefield_amd.capi import FP64
 
     stream = torch.cuda.ExternalStream(ctx.stream())
@@ -66,13 +73,37 @@ def time_kernel(ctx, torch, block_id, which, scaling, d_src, d_dst, reps):
     ctx.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(reps):
-        ctx.apply_operator(block_id, which, scaling, FP64, src, d_dst.data_ptr())
-    stop.record(stream)
-    stop.synchronize()
+    if flush is None:
+        start.record(stream)
+        for _ in range(reps):
+            ctx.apply_operator(block_id, which, scaling, FP64, src, d_dst.data_ptr())
+        stop.record(stream)
+        stop.synchronize()
+        total = start.elapsed_time(stop)
+    else:
+        total = 0.0
+        for r in range(reps):
+            with torch.cuda.stream(stream):
+                flush.fill_(float(r))
+            start.record(stream)
+            ctx.apply_operator(block_id, which, scaling, FP64, src, d_dst.data_ptr())
+            stop.record(stream)
+            stop.synchronize()
+            total += start.elapsed_time(stop)
     ctx.set_option("synchronize", "1")
-    return start.elapsed_time(stop) / reps * 1e-3  # seconds
+    return total / reps * 1e-3  # seconds
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE scaled by the copy-kernel calibration); None if the summary is absent."""
+    path = Path(__file__).resolve().parent / "profiles" / name
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(steps_target_s: float):
@@ -105,7 +136,7 @@ def cpu_baseline(steps_target_s: float):
 def poisson_extra(torch):
     """Poisson 128^3 Q2: FP64 operator roofline and GMG-CG time-to-solution."""
     from phasefield_amd import Context, prm
-    from phasefield_amd.capi import AGE_LATEST, OP_LHS, OP_RHS, SCALE_NONE
+    from phasefield_amd.capi import OP_LHS, OP_RHS, SCALE_NONE
 
     out = {}
     text = prm.poisson(dim=3, refinement=7, mg_depth=7, tolerance=1e-8, coarse_solver="cg",
@@ -123,32 +154,39 @@ def poisson_extra(torch):
     t_apply = time_kernel(ctx, torch, 1, OP_LHS, SCALE_NONE, x, y, 20)
     bytes_apply = 2 * 8 * n_d
     out["fp64_apply"] = {
-        "workload": "Poisson Q2 128^3 lhs application (grad.grad), FP64",
+        "workload": "Poisson Q2 128^3 lhs application (grad.grad), FP64, zero Dirichlet",
+        "kernel": "stencil_operator_kernel (Poisson lhs block, Q2, FP64)",
         "dofs": n_d,
         "seconds": t_apply,
         "dofs_per_s": n_d / t_apply,
+        "bytes_per_launch": bytes_apply,
         "achieved_gbs": bytes_apply / t_apply / 1e9,
         "frac": bytes_apply / t_apply / 1e9 / HBM_PEAK_GBS,
+        "traffic": profiled_traffic("stencil_poisson_q2_128_fp64_pmc.md"),
     }
     del x, y
-    for label, precision in (("fp64_levels", "double"), ("fp32_levels", "single")):
-        c = Context("poisson_3d_q2", text)
-        c.set_option("linear_tolerance", 1e-8 * bnorm)
-        c.set_option("mg_level_precision", precision)
-        c.solve_block(0)
-        c.set_time(1, 0.0)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        st = c.linear_solve(1)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        out[label] = {
-            "seconds_to_1e-8": dt,
-            "iterations": st.iterations,
-            "relative_residual": st.final_residual / bnorm,
-            "converged": bool(st.converged),
-        }
-        del c
+    for rel_tol in (1e-8, 1e-10):
+        res = {}
+        for label, precision in (("fp64_levels", "double"), ("fp32_levels", "single")):
+            c = Context("poisson_3d_q2", text)
+            c.set_option("linear_tolerance", rel_tol * bnorm)
+            c.set_option("mg_level_precision", precision)
+            c.solve_block(0)
+            c.set_time(1, 0.0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = c.linear_solve(1)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            res[label] = {
+                "seconds": dt,
+                "iterations": st.iterations,
+                "relative_residual": st.final_residual / bnorm,
+                "converged": bool(st.converged),
+            }
+            del c
+        res["fp32_levels_speedup"] = res["fp64_levels"]["seconds"] / res["fp32_levels"]["seconds"]
+        out[f"gmg_cg_to_{rel_tol:g}"] = res
     del ctx
     return out
 
@@ -189,6 +227,11 @@ def main():
     # Dominant kernel: the c-block operator (reads c_old and mu_old, writes c).
     dst = torch.zeros(info.n_nodes, dtype=torch.float64, device="cuda")
     t_kernel = time_kernel(ctx, torch, 0, OP_RHS, SCALE_INVM, None, dst, args.kernel_reps)
+    # Same kernel with L2 and Infinity Cache flushed before every launch (the 51.5 MB
+    # working set of this workload otherwise stays in the 256 MB Infinity Cache).
+    flush = torch.empty(128 << 20, dtype=torch.float64, device="cuda")  # 1 GiB
+    t_flushed = time_kernel(ctx, torch, 0, OP_RHS, SCALE_INVM, None, dst, 20, flush=flush)
+    del flush
     alg_bytes = 3 * 8 * n_d
     achieved = alg_bytes / t_kernel / 1e9
     roofline = {
@@ -197,14 +240,24 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
-        "kernel": "split_operator_kernel (Cahn-Hilliard c block, Q2, FP64)",
+        "traffic": profiled_traffic("stencil_cahn_hilliard_q2_64_fp64_pmc.md"),
+        "kernel": "stencil_operator_kernel (Cahn-Hilliard c block, Q2, FP64)",
         "launch_seconds": t_kernel,
         "bytes_per_launch": alg_bytes,
+        "achieved_flushed": alg_bytes / t_flushed / 1e9,
+        "frac_flushed": alg_bytes / t_flushed / 1e9 / HBM_PEAK_GBS,
+        "launch_seconds_flushed": t_flushed,
     }
     del ctx, dst
     torch.cuda.empty_cache()
 
+    if args.copy_calibration:
+        src = torch.randn(16974593, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     extra = None if args.no_extra else poisson_extra(torch)
     cpu = cpu_baseline(args.cpu_seconds) if args.cpu_seconds > 0 else None
 

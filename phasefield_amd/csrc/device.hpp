@@ -119,6 +119,40 @@ namespace ppf
     int64_t n_   = 0;
   };
 
+  /// Pinned host memory that kernels can write directly (status words read by the host
+  /// after a stream synchronization, without a separate copy).
+  template <typename T>
+  class PinnedBuffer
+  {
+  public:
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer &) = delete;
+    PinnedBuffer &operator=(const PinnedBuffer &) = delete;
+    ~PinnedBuffer()
+    {
+      if (ptr_)
+        (void) hipHostFree(ptr_);
+    }
+
+    T *
+    get(int64_t n = 1)
+    {
+      if (!ptr_)
+        {
+          PPF_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&ptr_), sizeof(T) * size_t(n),
+                                      hipHostMallocCoherent));
+          n_ = n;
+        }
+      if (n > n_)
+        throw Error(status_internal, "pinned buffer too small");
+      return ptr_;
+    }
+
+  private:
+    T      *ptr_ = nullptr;
+    int64_t n_   = 0;
+  };
+
   inline unsigned int
   grid_for(int64_t n, int threads)
   {

@@ -1583,18 +1583,20 @@ namespace ppf
             continue;
           solve_block(int(b));
         }
-      int status = 0;
-      for (std::size_t f = 0; f < fields.size() && status == 0; ++f)
-        {
-          hipLaunchKernelGGL(finite_probe_kernel<double>, dim3(1), dim3(1), 0, stream,
-                             reinterpret_cast<int *>(scalar_buf.get() + 1), field_ptr(int(f), 0));
-          int flag = 0;
-          PPF_HIP_CHECK(hipMemcpyAsync(&flag, scalar_buf.get() + 1, sizeof(int),
-                                       hipMemcpyDeviceToHost, stream));
-          sync();
-          if (flag)
-            status = 2;
-        }
+      // One probe of entry 0 of every field, written straight to pinned host memory;
+      // the stream synchronization below is the increment's only host wait.
+      int         status = 0;
+      ProbeFields pf {};
+      pf.n = int(fields.size());
+      for (std::size_t f = 0; f < fields.size(); ++f)
+        pf.ptr[f] = field_ptr(int(f), 0);
+      int *first_bad = probe_flag.get();
+      *first_bad     = -1;
+      hipLaunchKernelGGL(finite_probe_fields_kernel, dim3(1), dim3(1), 0, stream, first_bad, pf);
+      PPF_HIP_CHECK(hipGetLastError());
+      sync();
+      if (*reinterpret_cast<volatile int *>(first_bad) >= 0)
+        status = 2;
       if (status == 0 && pde_d->get_user_stop())
         status = 3;
       if (do_output || status != 0)
@@ -1683,6 +1685,7 @@ namespace ppf
     DeviceBuffer<float>                            scratch_f;
     DeviceBuffer<double>                           reduce_buf;
     DeviceBuffer<double>                           scalar_buf;
+    PinnedBuffer<int>                              probe_flag;
     std::string                                    log;
     std::array<bool, PPF_MAX_FIELDS>               rotated {};
   };

@@ -186,6 +186,27 @@ namespace ppf
       flag[0] = isfinite(double(x[0])) ? 0 : 1;
   }
 
+  /// Entry-0 NaN/Inf probe of every field at once: writes the index of the first field
+  /// whose entry 0 is not finite, or -1.
+  struct ProbeFields
+  {
+    const double *ptr[PPF_MAX_FIELDS];
+    int           n;
+  };
+
+  __global__ void
+  finite_probe_fields_kernel(int *__restrict__ first_bad, const ProbeFields fields)
+  {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+      {
+        int bad = -1;
+        for (int f = 0; f < fields.n && f < PPF_MAX_FIELDS; ++f)
+          if (bad < 0 && !isfinite(fields.ptr[f][0]))
+            bad = f;
+        *first_bad = bad;
+      }
+  }
+
   /// Output diagnostics: partial sums of u^2 over deal.II's full node set (periodic
   /// images counted with their multiplicity) and of u * jxw (images carry jxw = 0).
   template <int dim, int degree, typename T>
