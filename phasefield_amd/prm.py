@@ -77,10 +77,10 @@ def cahn_hilliard(size=64.0, refinement=6, steps=200, dt=1e-4, M=1.0, kappa=1.0,
 
 def poisson(dim=3, refinement=7, mg_depth=7, tolerance=1e-10, preconditioner="GMG",
             smoother_degree=4, max_iterations=1000, level_precision="double",
-            coarse_solver="cg", size=1.0, tolerance_type="AbsoluteResidual"):
+            coarse_solver="cg", size=1.0, tolerance_type="AbsoluteResidual", subdivisions=None):
     faces = ", ".join(["Dirichlet"] * (2 * dim))
     return "\n".join([
-        mesh_block(dim, size, refinement, periodic=False),
+        mesh_block(dim, size, refinement, subdivisions, periodic=False),
         _common(0, 0.0),
         "subsection boundary conditions: 0",
         "  set variables = u",

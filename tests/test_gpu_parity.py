@@ -10,7 +10,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from phasefield_amd import Context, prm
+from phasefield_amd import Context, PPFError, prm
 from phasefield_amd.capi import (AGE_LATEST, FP32, FP64, OP_LHS, OP_LHS_PLAIN, OP_RHS,
                                  SCALE_INVM, SCALE_INVM_SQRT, SCALE_NONE)
 from oracle.binding import Oracle
@@ -61,8 +61,11 @@ OPERATOR_CASES = [
 
 
 @pytest.mark.parametrize("app,text,ops", OPERATOR_CASES, ids=[c[0] for c in OPERATOR_CASES])
-@pytest.mark.parametrize("path", ["auto", "generic"])
+@pytest.mark.parametrize("path", ["auto", "generic", "split_node"])
 def test_operator_matches_oracle(app, text, ops, path):
+    """auto: tiled stencil kernel for split hooks, generic cell path otherwise;
+    generic: cell path for every hook; split_node: one-thread-per-node split kernel
+    (hooks without the split structure are rejected with an error)."""
     ctx, orc = Context(app, text), Oracle(app, text)
     info = ctx.info
     rng = np.random.default_rng(7)
@@ -77,11 +80,56 @@ def test_operator_matches_oracle(app, text, ops, path):
             ref = orc.apply_operator(block_id, which, scaling, src_full, nb)
             d_src = dev(src_stored)
             d_dst = torch.zeros_like(d_src)
-            ctx.apply_operator(block_id, which, scaling, FP64, d_src.data_ptr(), d_dst.data_ptr())
+            if path == "split_node":
+                try:
+                    ctx.apply_operator(block_id, which, scaling, FP64, d_src.data_ptr(),
+                                       d_dst.data_ptr())
+                except PPFError as e:
+                    assert "does not qualify" in str(e)
+                    continue
+            else:
+                ctx.apply_operator(block_id, which, scaling, FP64, d_src.data_ptr(), d_dst.data_ptr())
             got = d_dst.cpu().numpy()
             err = rel_err(got, block_to_stored(ref, info, nb))
             print(app, path, block_id, which, scaling, err, flush=True)
             assert err <= 1e-12, (block_id, which, scaling, err)
+
+
+# Tile shapes of the stencil kernel: (rows per wave, z-planes per workgroup; 0 = automatic).
+TILINGS = [(2, 0), (4, 0), (2, 5), (4, 7), (2, 64), (4, 1)]
+TILING_CASES = [
+    # 257 x 33 x 33 nodes, Dirichlet on every face: tiles in the interior run the
+    # variant without face tests, the others the masked variant.
+    ("poisson_3d_q2", prm.poisson(dim=3, refinement=4, mg_depth=1, subdivisions=[8, 1, 1]),
+     [(1, OP_LHS, 1), (1, OP_LHS_PLAIN, 1), (1, OP_RHS, 1)]),
+    # Periodic lattice of 4 nodes per direction (2 cells): ring wrap-around within a tile.
+    ("cahn_hilliard_3d_q2", prm.cahn_hilliard(size=4.0, refinement=1),
+     [(0, OP_RHS, 1), (1, OP_RHS, 1)]),
+    ("allen_cahn_2d_q1", prm.allen_cahn(dim=2, size=30.0, refinement=5, subdivisions=[5, 3]),
+     [(1, OP_RHS, 1)]),
+]
+
+
+@pytest.mark.parametrize("app,text,ops", TILING_CASES, ids=[c[0] for c in TILING_CASES])
+def test_stencil_tilings_match_oracle(app, text, ops):
+    ctx, orc = Context(app, text), Oracle(app, text)
+    info = ctx.info
+    rng = np.random.default_rng(21)
+    seed_fields(ctx, orc, rng)
+    for rows, zc in TILINGS:
+        ctx.set_option("stencil_rows", rows)
+        ctx.set_option("stencil_z_chunk", zc)
+        for block_id, which, nb in ops:
+            for scaling in (SCALE_NONE, SCALE_INVM, SCALE_INVM_SQRT):
+                src_stored = rng.uniform(-1, 1, nb * info.n_nodes)
+                ref = orc.apply_operator(block_id, which, scaling,
+                                         block_to_full(src_stored, info, nb), nb)
+                d_src = dev(src_stored)
+                d_dst = torch.zeros_like(d_src)
+                ctx.apply_operator(block_id, which, scaling, FP64, d_src.data_ptr(),
+                                   d_dst.data_ptr())
+                err = rel_err(d_dst.cpu().numpy(), block_to_stored(ref, info, nb))
+                assert err <= 1e-12, (rows, zc, block_id, which, scaling, err)
 
 
 @pytest.mark.parametrize("app,text,ops", OPERATOR_CASES[2:6], ids=[c[0] for c in OPERATOR_CASES[2:6]])
