@@ -58,6 +58,13 @@ namespace ppf
     int    n[3];        // nodes per axis (n[2] = 1 for 2-D lattices)
     int    periodic[3];
     int    tiles_x, tiles_y, chunks_z, z_chunk, n_blocks;
+    /// Tiles cover x < tile_n[0], y < tile_n[1]; the rest of the lattice (a few columns
+    /// or rows left over by the tile widths) is processed node by node by the
+    /// workgroups after the n_blocks tile workgroups: box A = [tile_n[0], n0) x [0, n1)
+    /// x [0, n2) with rem_a nodes, then box B = [0, tile_n[0]) x [tile_n[1], n1) x
+    /// [0, n2) with rem_b nodes.
+    int     tile_n[2];
+    int64_t rem_a, rem_b;
     /// Taps (centre, +-1, +-2) per axis and coordinate parity (index: i & 1 for Q2,
     /// both equal for Q1), already divided by h^2.
     number tap[3][2][3];
@@ -76,7 +83,8 @@ namespace ppf
     unsigned int  val_dmask[PPF_MAX_FIELDS][PPF_N_SLOTS];
     /// Operand index k if the input is operand k (read from the ring), -1 otherwise.
     /// Every `op` and `val` pointer is valid (the host points unused entries at an
-    /// array of the lattice size), so loads need no branch.
+    /// array of the lattice size, and an absent operand at an array of zeros), so
+    /// loads need no branch.
     signed char   val_alias[PPF_MAX_FIELDS][PPF_N_SLOTS];
     /// Union of all face masks the operator reads or writes (0: no face test at all).
     unsigned int  any_dmask;
@@ -134,6 +142,104 @@ namespace ppf
     return !periodic && (lo - rad <= 0 || hi + rad >= n - 1);
   }
 
+  /// Operand s at lattice coordinate (x, y, z), which may lie a stencil radius outside
+  /// the lattice (source node by wrap / mirror), zero on the operand's Dirichlet faces.
+  template <typename number>
+  __device__ PPF_INLINE number
+  operand_at(const StencilArgs<number> &a, int s, int x, int y, int z)
+  {
+    const int xs = source_coord(x, a.n[0], a.periodic[0]);
+    const int ys = source_coord(y, a.n[1], a.periodic[1]);
+    const int zs = source_coord(z, a.n[2], a.periodic[2]);
+    const unsigned int fb = face_bits(xs, a.n[0], a.periodic[0], 0) |
+                            face_bits(ys, a.n[1], a.periodic[1], 1) |
+                            face_bits(zs, a.n[2], a.periodic[2], 2);
+    const number v =
+      a.op[s][int64_t(xs) + int64_t(a.n[0]) * (int64_t(ys) + int64_t(a.n[1]) * zs)];
+    return (fb & a.op_dmask[s]) ? number(0) : v;
+  }
+
+  /// The operator at one node, evaluated directly from global memory (the remainder
+  /// workgroups of stencil_operator_kernel; same arithmetic as the tiles).
+  template <class PDE, int dim, int degree, typename number, bool lhs, int block_id, int NF,
+            int NS>
+  __device__ void
+  stencil_node(const PDE &pde, const StencilArgs<number> &a, const OutTable<number> &out,
+               const prismspf::SimulationTimer &timer, int x, int y, int z)
+  {
+    constexpr int RAD = degree;
+    const int     c[3] = {x, y, z};
+    number        tap[3][3], sc = number(1);
+    unsigned int  fb = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      {
+        const bool odd = degree == 2 && (c[d] & 1);
+        for (int k = 0; k < 3; ++k)
+          tap[d][k] = odd ? a.tap[d][1][k] : a.tap[d][0][k];
+        const int cl = scale_class<degree>(c[d], a.n[d], a.periodic[d]);
+        sc *= cl == 0 ? a.sfac[d][0] : (cl == 1 ? a.sfac[d][1] : a.sfac[d][2]);
+        fb |= face_bits(c[d], a.n[d], a.periodic[d], d);
+      }
+    number lap[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      {
+        number acc = (tap[0][0] + tap[1][0] + (dim == 3 ? tap[2][0] : number(0))) *
+                     operand_at(a, s, x, y, z);
+#pragma unroll
+        for (int o = 1; o <= RAD; ++o)
+          {
+            acc += tap[0][o] * (operand_at(a, s, x - o, y, z) + operand_at(a, s, x + o, y, z));
+            acc += tap[1][o] * (operand_at(a, s, x, y - o, z) + operand_at(a, s, x, y + o, z));
+            if constexpr (dim == 3)
+              acc += tap[2][o] * (operand_at(a, s, x, y, z - o) + operand_at(a, s, x, y, z + o));
+          }
+        lap[s] = acc;
+      }
+    const int64_t node = int64_t(x) + int64_t(a.n[0]) * (int64_t(y) + int64_t(a.n[1]) * z);
+    prismspf::FieldContainer<dim, degree, number> fc;
+    fc.q_point[0] = number(a.q_lower[0] + x * a.q_step[0]);
+    fc.q_point[1] = number(a.q_lower[1] + y * a.q_step[1]);
+    if constexpr (dim == 3)
+      fc.q_point[2] = number(a.q_lower[2] + z * a.q_step[2]);
+    fc.element_volume = number(a.element_volume);
+#pragma unroll
+    for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+#pragma unroll
+      for (int s = 0; s < PPF_N_SLOTS; ++s)
+        {
+          number v = number(0);
+          if (f < NF)
+            {
+              const int fi = f < NF ? f : 0;
+              const int al = a.val_alias[fi][s];
+              v = (fb & a.val_dmask[fi][s]) ? number(0) : a.val[fi][s][node];
+              if (al >= 0)
+                v = operand_at(a, al < NS ? al : 0, x, y, z);
+            }
+          fc.value[f][s] = v;
+#pragma unroll
+          for (int d = 0; d < dim; ++d)
+            fc.gradient[f][s][d] = number(0);
+        }
+    fc.clear_terms();
+    call_hook<PDE, dim, degree, number, lhs, block_id>(pde, fc, timer);
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+      {
+        if (!(out.out_mask & (1u << f)))
+          continue;
+        number acc = (fc.value_term_mask & (1u << f)) ? fc.value_term[f] : number(0);
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+          acc += a.beta[f][k] * lap[k];
+        const bool dir = (fb & out.dirichlet_mask[f]) != 0;
+        if (!(dir && out.dirichlet_write == dirichlet_value))
+          out.dst[f][node] = dir ? number(0) : sc * acc;
+      }
+  }
+
   template <class PDE, int dim, int degree, typename number, bool lhs, int block_id, int NF,
             int R, int NS>
   __global__ void __launch_bounds__(stencil_threads)
@@ -157,6 +263,35 @@ namespace ppf
     const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
     const int p0 = a.periodic[0], p1 = a.periodic[1], p2 = a.periodic[2];
 
+    if (int(blockIdx.x) >= a.n_blocks)
+      {
+        // Remainder workgroup: one node per thread.
+        int64_t t = int64_t(int(blockIdx.x) - a.n_blocks) * stencil_threads +
+                    threadIdx.y * stencil_tx + threadIdx.x;
+        int x, y, z;
+        if (t < a.rem_a)
+          {
+            const int w = n0 - a.tile_n[0];
+            x           = a.tile_n[0] + int(t % w);
+            t /= w;
+            y = int(t % n1);
+            z = int(t / n1);
+          }
+        else if (t < a.rem_a + a.rem_b)
+          {
+            t -= a.rem_a;
+            const int w = a.tile_n[0], hgt = n1 - a.tile_n[1];
+            x           = int(t % w);
+            t /= w;
+            y = a.tile_n[1] + int(t % hgt);
+            z = int(t / hgt);
+          }
+        else
+          return;
+        stencil_node<PDE, dim, degree, number, lhs, block_id, NF, NS>(pde, a, out, timer, x, y, z);
+        return;
+      }
+
     const int b   = blockIdx.x;
     const int q8  = a.n_blocks / 8;
     const int lb  = b < 8 * q8 ? (b % 8) * q8 + b / 8 : b;
@@ -172,7 +307,7 @@ namespace ppf
 
     // Own column (lane-constant).
     const int          x    = x0 + tx;
-    const bool         xout = x < n0; // owns output nodes
+    const bool         xout = x < a.tile_n[0]; // owns output nodes
     const int          xs   = source_coord(x, n0, p0);
     const unsigned int fbx  = face_bits(xs, n0, p0, 0);
     const int          pxi  = degree == 2 ? (xs & 1) : 0;
@@ -191,7 +326,7 @@ namespace ppf
     for (int r = 0; r < R; ++r)
       {
         const int y = y0 + w + stencil_waves * r;
-        yout[r]     = y < n1;
+        yout[r]     = y < a.tile_n[1];
         ys[r]       = source_coord(y, n1, p1);
         fby[r]      = face_bits(ys[r], n1, p1, 1);
         own_off[r]  = int64_t(xs) + int64_t(n0) * ys[r];
@@ -246,6 +381,16 @@ namespace ppf
             ring_v[slot][r][f][s] = a.val[f][s][po + own_off[r]];
     };
 
+    // Zeroes the operands of ring slot k on their Dirichlet faces.
+    auto mask_slot = [&](int k) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          ring[s][r][k] = ((fbx | fby[r] | ring_fb[k]) & a.op_dmask[s]) ? number(0)
+                                                                       : ring[s][r][k];
+    };
+
     // One plane.  MASKED: face tests (constrained reads, Dirichlet outputs); without
     // it the tile's read range contains no boundary node of a masked face.
     auto step = [&]<int PH, bool MASKED>(std::integral_constant<int, PH>,
@@ -259,15 +404,13 @@ namespace ppf
           load_inputs(min(z + D, ze - 1), ns);
         }
 
-      // Operand value of ring slot k, row r, zeroed on its Dirichlet faces in masked
-      // tiles.  (Entries of absent operands hold arbitrary values; their sums are
-      // discarded below.)
-      auto opv = [&](int s, int r, int k) {
-        if constexpr (MASKED)
-          return ((fbx | fby[r] | ring_fb[k]) & a.op_dmask[s]) ? number(0) : ring[s][r][k];
-        else
-          return ring[s][r][k];
-      };
+      // In masked tiles the plane entering the z taps (z + RAD) is zeroed on the
+      // operands' Dirichlet faces once, in place; every other ring plane in use has
+      // been zeroed before (or by the prologue).
+      if constexpr (MASKED && dim == 3)
+        mask_slot((cs + RAD) % S);
+      else if constexpr (MASKED)
+        mask_slot(cs);
 
       // Stage the centre plane.
       number ctr[NS][R];
@@ -277,7 +420,7 @@ namespace ppf
 #pragma unroll
           for (int r = 0; r < R; ++r)
             {
-              ctr[s][r] = opv(s, r, cs);
+              ctr[s][r] = ring[s][r][cs];
               lds[buf][s][RAD + w + stencil_waves * r][RAD + tx] = ctr[s][r];
             }
           number hyv = ring_hy[s][cs], hxv = ring_hx[s][cs];
@@ -330,15 +473,15 @@ namespace ppf
               acc += wx1 * (L[ly][lx - 1] + L[ly][lx + 1]);
               acc += wy1 * (L[ly - 1][lx] + L[ly + 1][lx]);
               if constexpr (dim == 3)
-                acc += wz1 * (opv(s, r, (cs + S - 1) % S) + opv(s, r, (cs + 1) % S));
+                acc += wz1 * (ring[s][r][(cs + S - 1) % S] + ring[s][r][(cs + 1) % S]);
               if constexpr (RAD == 2)
                 {
                   acc += wx2 * (L[ly][lx - 2] + L[ly][lx + 2]);
                   acc += wy2 * (L[ly - 2][lx] + L[ly + 2][lx]);
                   if constexpr (dim == 3)
-                    acc += wz2 * (opv(s, r, (cs + S - 2) % S) + opv(s, r, (cs + 2) % S));
+                    acc += wz2 * (ring[s][r][(cs + S - 2) % S] + ring[s][r][(cs + 2) % S]);
                 }
-              lap[s] = s < a.n_ops ? acc : number(0);
+              lap[s] = acc;
             }
 
           // Hook at the node; lanes and rows without an output node evaluate it on
@@ -413,6 +556,13 @@ namespace ppf
 #pragma unroll
           for (int k = RAD; k < T; ++k)
             load_inputs(min(zb - RAD + k, ze - 1), k);
+          if constexpr (MASKED)
+            {
+              // planes zb-RAD .. zb+RAD-1 never enter as the newest z tap
+#pragma unroll
+              for (int k = 0; k < 2 * RAD; ++k)
+                mask_slot(k);
+            }
           for (int z = zb; z < ze; z += S)
             {
               [&]<int... PH>(std::integer_sequence<int, PH...>) {
