@@ -209,16 +209,17 @@ def main():
     info = ctx.info
     n_d = info.n_dofs
 
-    ctx.solve_increment()  # increment 0: initial condition of c, mu from it
-    for _ in range(args.warmup):
-        ctx.solve_increment()
+    # Increment 0 (initial condition of c, mu from it) and the warmup increments.
+    if ctx.run(args.warmup) != 0:
+        raise RuntimeError("warmup increments stopped early")
     torch.cuda.synchronize()
     ctx.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        status = ctx.solve_increment()
-        if status != 0:
-            raise RuntimeError(f"increment loop stopped with status {status}")
+    # Increments warmup+1 .. warmup+steps through ppf_run (Problem::solve's loop, NaN
+    # probe after every increment).
+    status = ctx.run(args.warmup + args.steps)
+    if status != 0 or ctx.get_info().increment != args.warmup + args.steps + 1:
+        raise RuntimeError(f"increment loop stopped with status {status}")
     ctx.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0

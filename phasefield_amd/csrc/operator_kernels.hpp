@@ -58,7 +58,16 @@ namespace ppf
     unsigned int  out_mask;
     int           scaling;
     int           dirichlet_write;
+    /// Device flag that, once nonzero, turns the launch into a no-op (an increment
+    /// enqueued ahead of the previous increment's NaN probe); null: always run.
+    const int    *halt;
   };
+
+  __device__ inline bool
+  halted(const int *halt)
+  {
+    return halt && *halt;
+  }
 
   /// One gradient-stencil term of the split path: dst[out] += beta * K u[field][slot].
   struct SplitTerm
@@ -133,7 +142,7 @@ namespace ppf
     using Basis       = GLL<degree>;
 
     const int64_t cell = int64_t(blockIdx.x) * block_size + threadIdx.x;
-    if (cell >= g.n_cells)
+    if (cell >= g.n_cells || halted(out.halt))
       return;
     int c[3];
     decompose<dim>(g, cell, g.cells, c);
@@ -330,7 +339,7 @@ namespace ppf
   {
     constexpr int NPC  = dofs_per_cell<dim, degree>;
     const int64_t node = int64_t(blockIdx.x) * block_size + threadIdx.x;
-    if (node >= g.n_nodes)
+    if (node >= g.n_nodes || halted(out.halt))
       return;
     int i[3];
     decompose<dim>(g, node, g.nodes, i);
@@ -434,7 +443,7 @@ namespace ppf
                         const prismspf::SimulationTimer timer)
   {
     const int64_t node = int64_t(blockIdx.x) * block_size + threadIdx.x;
-    if (node >= g.n_nodes)
+    if (node >= g.n_nodes || halted(out.halt))
       return;
     int i[3];
     decompose<dim>(g, node, g.nodes, i);

@@ -158,6 +158,7 @@ namespace ppf
       PPF_HIP_CHECK(hipStreamCreate(&stream));
       reduce_buf.resize(2 * reduce_blocks + 8);
       scalar_buf.resize(8);
+      halt_flag.resize(1);
       setup_storage();
       compute_dirichlet_values(fine, dirichlet_values);
     }
@@ -542,6 +543,7 @@ namespace ppf
         }
       out.scaling         = scaling;
       out.dirichlet_write = dwrite;
+      out.halt            = enqueue_halt;
 
       const SplitPlan &plan = split_plan<number>(b, lhs);
       const bool use_split  = path == OperatorPath::Split || path == OperatorPath::SplitNode ||
@@ -687,7 +689,8 @@ namespace ppf
           if (out.out_mask & (1u << f))
             a.any_dmask |= out.dirichlet_mask[f];
         }
-      if (int64_t(a.n[0]) * a.n[1] >= (int64_t(1) << 31))
+      // 32-bit byte offsets in the kernel: arrays up to 4 GiB.
+      if (int64_t(a.n[0]) * a.n[1] * a.n[2] * int64_t(sizeof(number)) > (int64_t(1) << 32))
         return false;
 
       int rows = stencil_rows;
@@ -721,6 +724,10 @@ namespace ppf
               zc = int((int64_t(a.n[2]) * xy + stencil_target_blocks - 1) / stencil_target_blocks);
               zc = std::max(zc, 4 * degree);
             }
+          // Q2 chunks start at even z (the kernel takes the z taps' parity from the
+          // step's phase).
+          if (degree == 2 && (zc & 1))
+            ++zc;
           a.z_chunk  = std::min(zc, a.n[2]);
           a.chunks_z = (a.n[2] + a.z_chunk - 1) / a.z_chunk;
         }
@@ -771,7 +778,7 @@ namespace ppf
                                  dim3(unsigned((face + 255) / 256), 6), dim3(256), 0, stream,
                                  out.dst[f], out.dirichlet_values[f], a.n[0], a.n[1], a.n[2],
                                  a.periodic[0], a.periodic[1], a.periodic[2],
-                                 out.dirichlet_mask[f]);
+                                 out.dirichlet_mask[f], out.halt);
             }
       return true;
     }
@@ -1603,13 +1610,31 @@ namespace ppf
       return rotated[f] ? 1 : 0;
     }
 
-    /// Problem::solve_increment (reference problem.cc:414-558).  Returns the exit status:
-    /// 0 normal, 2 NaN detected, 3 user stop.
-    int
-    solve_increment()
+    void
+    rotate_all()
     {
-      const unsigned int inc       = timer.get_increment();
-      const bool         do_output = inputs.output_parameters.should_output(inc);
+      for (std::size_t b = 0; b < blocks.size(); ++b)
+        rotate(int(b));
+    }
+
+    /// Inverse of rotate_all (ages shifted back by one).
+    void
+    unrotate_all()
+    {
+      for (const auto &blk : blocks)
+        for (auto f : blk.field_indices)
+          for (int age = 1; age < int(storage[f].size()); ++age)
+            storage[f][age].swap(storage[f][age - 1]);
+    }
+
+    /// Enqueues the solve blocks of the current increment, then the NaN probe of
+    /// entry 0 of every field, whose result goes to pinned slot `slot`; no host wait.
+    /// With `halt` given, every launch carries the flag (and does nothing once an
+    /// earlier probe has set it) and the probe sets it on a non-finite entry.
+    void
+    enqueue_increment(bool do_output, int slot, int *halt)
+    {
+      enqueue_halt = halt;
       for (std::size_t b = 0; b < blocks.size(); ++b)
         {
           const auto timing = blocks[b].solve_timing;
@@ -1618,39 +1643,161 @@ namespace ppf
             continue;
           solve_block(int(b));
         }
-      // One probe of entry 0 of every field, written straight to pinned host memory;
-      // the stream synchronization below is the increment's only host wait.
-      int         status = 0;
+      enqueue_halt = nullptr;
       ProbeFields pf {};
       pf.n = int(fields.size());
       for (std::size_t f = 0; f < fields.size(); ++f)
         pf.ptr[f] = field_ptr(int(f), 0);
-      int *first_bad = probe_flag.get();
-      *first_bad     = -1;
-      hipLaunchKernelGGL(finite_probe_fields_kernel, dim3(1), dim3(1), 0, stream, first_bad, pf);
+      probe_tag[slot] = int(++probe_count & 0x3ffffffu);
+      int *result     = probe_flag.get(2) + slot;
+      *reinterpret_cast<volatile int *>(result) = -1;
+      hipLaunchKernelGGL(finite_probe_fields_kernel, dim3(1), dim3(1), 0, stream, result, pf,
+                         halt, probe_tag[slot]);
       PPF_HIP_CHECK(hipGetLastError());
-      sync();
-      if (*reinterpret_cast<volatile int *>(first_bad) >= 0)
-        status = 2;
+    }
+
+    /// Waits for the probe result in `slot` by polling its pinned word (no event or
+    /// stream synchronization, whose completion markers would cost the GPU several
+    /// microseconds per increment); true if the probe found a non-finite entry.
+    bool
+    probe_found_nan(int slot)
+    {
+      const volatile int *word = probe_flag.get(2) + slot;
+      const int           tag  = probe_tag[slot];
+      for (unsigned long spin = 1;; ++spin)
+        {
+          const int v = *word;
+          if ((v >> 5) == tag)
+            return (v & 31) != 0;
+          if (spin % 4096 == 0)
+            {
+              // Ends the wait if the stream has failed or drained without the result.
+              const hipError_t q = hipStreamQuery(stream);
+              if (q == hipSuccess)
+                {
+                  const int w = *word;
+                  if ((w >> 5) != tag)
+                    throw Error(status_internal, "NaN probe result missing");
+                  return (w & 31) != 0;
+                }
+              if (q != hipErrorNotReady)
+                PPF_HIP_CHECK(q);
+            }
+        }
+    }
+
+    /// Problem::solve_increment (reference problem.cc:414-558).  Returns the exit status:
+    /// 0 normal, 2 NaN detected, 3 user stop.  The probe's host wait is the increment's
+    /// only one.
+    int
+    solve_increment()
+    {
+      const bool do_output = inputs.output_parameters.should_output(timer.get_increment());
+      enqueue_increment(do_output, 0, nullptr);
+      int status = probe_found_nan(0) ? 2 : 0;
       if (status == 0 && pde_d->get_user_stop())
         status = 3;
       if (do_output || status != 0)
         log_norms();
-      for (std::size_t b = 0; b < blocks.size(); ++b)
-        rotate(int(b));
+      rotate_all();
       return status;
     }
 
-    /// Problem::solve (reference problem.cc:340-412).
+    /// Whether increment `inc` may be enqueued before the probe of the increment
+    /// before it has been read: it solves explicit blocks only (no host-side
+    /// convergence test), writes no output, and no user stop is set.
+    bool
+    speculable(unsigned int inc) const
+    {
+      if (inc == 0 || inputs.output_parameters.should_output(inc) || pde_d->get_user_stop())
+        return false;
+      for (const auto &blk : blocks)
+        {
+          const auto timing = blk.solve_timing;
+          if (timing == prismspf::PostProcess || timing == prismspf::NucleationRate)
+            continue; // skipped on increments without output
+          if (blk.solve_type != prismspf::Explicit)
+            return false;
+        }
+      return true;
+    }
+
+    /// Host-side state that an enqueued increment changes: the timer and the
+    /// rotation flags (the age order of the storage is undone by unrotate_all).
+    struct HostState
+    {
+      prismspf::SimulationTimer             timer;
+      std::array<bool, PPF_MAX_FIELDS>      rotated;
+    };
+
+    /// Problem::solve (reference problem.cc:340-412): increments until n_increments,
+    /// stopping after the first increment whose probe finds a non-finite entry
+    /// (status 2) or on a user stop (3).
+    ///
+    /// Runs of explicit increments without output are pipelined one deep: increment
+    /// k+1 is enqueued before increment k's probe is read, so the GPU never idles on
+    /// the host round trip.  Increment k+1's launches carry a device flag that
+    /// increment k's probe sets on a NaN, which makes them no-ops; the host then
+    /// undoes k+1's rotation and timer step.  Results, the log and the final state
+    /// are those of the one-at-a-time loop.
     int
     run(unsigned int n_increments)
     {
-      int status = 0;
-      while (timer.get_increment() <= n_increments && status == 0)
+      int  status = 0;
+      int *halt   = halt_flag.get();
+      hipLaunchKernelGGL(fill_kernel<int>, dim3(1), dim3(64), 0, stream, halt, 0, int64_t(1));
+      PPF_HIP_CHECK(hipGetLastError());
+      bool      pending = false; // an increment enqueued whose probe is unread
+      int       pending_slot = 0;
+      HostState pending_before {};
+      // NaN in the pending increment: back to the state before its rotation, log its
+      // norms, then rotate and step the timer as the one-at-a-time loop does.
+      auto settle_nan = [&](const HostState &before) {
+        unrotate_all();
+        timer   = before.timer;
+        rotated = before.rotated;
+        log_norms();
+        rotate_all();
+        timer.increment();
+        return 2;
+      };
+      while (status == 0 && timer.get_increment() <= n_increments)
         {
-          status = solve_increment();
+          const unsigned int inc = timer.get_increment();
+          if (!speculable(inc))
+            {
+              if (pending)
+                {
+                  pending = false;
+                  if (probe_found_nan(pending_slot))
+                    status = settle_nan(pending_before);
+                  continue;
+                }
+              status = solve_increment();
+              timer.increment();
+              continue;
+            }
+          const HostState before {timer, rotated};
+          const int       slot = int(inc & 1u);
+          enqueue_increment(false, slot, halt);
+          rotate_all();
           timer.increment();
+          if (pending && probe_found_nan(pending_slot))
+            {
+              // The increment just enqueued did nothing on the device.
+              unrotate_all();
+              timer   = before.timer;
+              rotated = before.rotated;
+              status  = settle_nan(pending_before);
+              pending = false;
+              break;
+            }
+          pending        = true;
+          pending_slot   = slot;
+          pending_before = before;
         }
+      if (pending && probe_found_nan(pending_slot))
+        status = settle_nan(pending_before);
       sync();
       return status;
     }
@@ -1722,7 +1869,11 @@ namespace ppf
     DeviceBuffer<float>                            zeros_f;
     DeviceBuffer<double>                           reduce_buf;
     DeviceBuffer<double>                           scalar_buf;
-    PinnedBuffer<int>                              probe_flag;
+    PinnedBuffer<int>                              probe_flag; // two probe slots
+    int                                            probe_tag[2] = {0, 0};
+    unsigned int                                   probe_count  = 0;
+    DeviceBuffer<int>                              halt_flag;
+    const int                                     *enqueue_halt = nullptr;
     std::string                                    log;
     std::array<bool, PPF_MAX_FIELDS>               rotated {};
   };

@@ -142,6 +142,39 @@ namespace ppf
     return !periodic && (lo - rad <= 0 || hi + rad >= n - 1);
   }
 
+  using buffer_rsrc = __amdgpu_buffer_rsrc_t;
+  typedef unsigned int u32x2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
+
+  /// Raw buffer resource over an array: loads and stores take a 32-bit per-lane byte
+  /// offset (kept in a register for the whole kernel) plus a scalar byte offset (the
+  /// plane), with no 64-bit address arithmetic.  No range limit is set (the callers'
+  /// arrays stay below 4 GiB); word 3 selects the 32-bit data format.
+  __device__ PPF_INLINE buffer_rsrc
+  make_rsrc(const void *p)
+  {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), short(0), -1, 0x00020000);
+  }
+
+  template <typename number>
+  __device__ PPF_INLINE number
+  buffer_load(buffer_rsrc r, unsigned int voff, unsigned int soff)
+  {
+    if constexpr (sizeof(number) == 8)
+      return __builtin_bit_cast(number, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    else
+      return __builtin_bit_cast(number, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  }
+
+  template <typename number>
+  __device__ PPF_INLINE void
+  buffer_store(number v, buffer_rsrc r, unsigned int voff, unsigned int soff)
+  {
+    if constexpr (sizeof(number) == 8)
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
+  }
+
   /// Operand s at lattice coordinate (x, y, z), which may lie a stencil radius outside
   /// the lattice (source node by wrap / mirror), zero on the operand's Dirichlet faces.
   template <typename number>
@@ -255,13 +288,15 @@ namespace ppf
     constexpr int LY  = TY + 2 * RAD;
     constexpr int D   = RAD + 1; // load distance in planes
     constexpr int S   = T + 1;   // ring length
-    __shared__ number lds[2][NS][LY][LX];
+    __shared__ number lds[2][NS][LY + 1][LX]; // row LY: spare target of idle lanes
 
     const int tx  = threadIdx.x;
     const int w   = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int tid = w * stencil_tx + tx;
     const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
     const int p0 = a.periodic[0], p1 = a.periodic[1], p2 = a.periodic[2];
+    if (halted(out.halt))
+      return;
 
     if (int(blockIdx.x) >= a.n_blocks)
       {
@@ -301,9 +336,23 @@ namespace ppf
     const int chk = t_r / a.tiles_y;
     const int x0  = t_x * stencil_tx;
     const int y0  = t_y * TY;
-    const int zb  = dim == 3 ? chk * a.z_chunk : 0;
+    const int zb  = dim == 3 ? chk * a.z_chunk : 0; // even for Q2 (host)
     const int ze  = dim == 3 ? min(zb + a.z_chunk, n2) : 1;
-    const int64_t plane_size = int64_t(n0) * n1;
+    // Byte offsets fit 32 bits: the host runs this kernel on arrays below 4 GiB only.
+    const unsigned int plane_bytes = unsigned(n0) * unsigned(n1) * unsigned(sizeof(number));
+
+    buffer_rsrc r_op[NS], r_val[NF][PPF_N_SLOTS], r_dst[NF];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      r_op[s] = make_rsrc(a.op[s]);
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+      {
+        r_dst[f] = make_rsrc(out.dst[f]);
+#pragma unroll
+        for (int s = 0; s < PPF_N_SLOTS; ++s)
+          r_val[f][s] = make_rsrc(a.val[f][s]);
+      }
 
     // Own column (lane-constant).
     const int          x    = x0 + tx;
@@ -317,11 +366,11 @@ namespace ppf
     const int          cx   = scale_class<degree>(xs, n0, p0);
     const number       sx   = cx == 0 ? a.sfac[0][0] : (cx == 1 ? a.sfac[0][1] : a.sfac[0][2]);
 
-    // Own rows (wave-uniform).
+    // Own rows (wave-uniform): source row, face bits, byte offset, taps, scale factor.
     int          ys[R];
     bool         yout[R];
-    unsigned int fby[R];
-    int64_t      own_off[R];
+    unsigned int fby[R], own_b[R];
+    number       wy0[R], wy1[R], wy2[R], sy[R];
 #pragma unroll
     for (int r = 0; r < R; ++r)
       {
@@ -329,11 +378,19 @@ namespace ppf
         yout[r]     = y < a.tile_n[1];
         ys[r]       = source_coord(y, n1, p1);
         fby[r]      = face_bits(ys[r], n1, p1, 1);
-        own_off[r]  = int64_t(xs) + int64_t(n0) * ys[r];
+        own_b[r]    = (unsigned(xs) + unsigned(n0) * unsigned(ys[r])) * unsigned(sizeof(number));
+        const bool py = degree == 2 && (ys[r] & 1);
+        wy0[r]        = py ? a.tap[1][1][0] : a.tap[1][0][0];
+        wy1[r]        = py ? a.tap[1][1][1] : a.tap[1][0][1];
+        wy2[r]        = py ? a.tap[1][1][2] : a.tap[1][0][2];
+        const int cy  = scale_class<degree>(ys[r], n1, p1);
+        sy[r]         = cy == 0 ? a.sfac[1][0] : (cy == 1 ? a.sfac[1][1] : a.sfac[1][2]);
       }
 
     // Halo assignments: wave w < 2 RAD loads one y-halo row of the tile's columns;
     // threads tid < 2 RAD TY load one x-halo node (the others repeat the y-halo load).
+    // Lanes without a halo node store to the spare LDS row, so staging has no branch.
+    __builtin_assume(w >= 0 && w < stencil_waves);
     const int          hy_row = w < RAD ? w : TY + w; // LDS row
     const bool         hy_on  = w < 2 * RAD;
     const int          hy_y   = source_coord(y0 - RAD + hy_row, n1, p1);
@@ -345,8 +402,16 @@ namespace ppf
     const int          hx_x   = source_coord(x0 - RAD + hx_col, n0, p0);
     const int          hx_y   = source_coord(y0 + hx_row, n1, p1);
     const unsigned int fb_hx  = face_bits(hx_x, n0, p0, 0) | face_bits(hx_y, n1, p1, 1);
-    const int64_t      hy_off = int64_t(xs) + int64_t(n0) * hy_y;
-    const int64_t      hx_off = hx_on ? int64_t(hx_x) + int64_t(n0) * hx_y : hy_off;
+    const unsigned int hy_b =
+      (unsigned(xs) + unsigned(n0) * unsigned(hy_y)) * unsigned(sizeof(number));
+    const unsigned int hx_b =
+      hx_on ? (unsigned(hx_x) + unsigned(n0) * unsigned(hx_y)) * unsigned(sizeof(number)) : hy_b;
+    const int hy_ly = hy_on ? hy_row : LY;
+    const int hx_ly = hx_on ? RAD + hx_row : LY;
+    const int hx_lx = hx_on ? hx_col : tx;
+
+    // Scale factors of the z classes (the taps follow the step's parity).
+    const number sz_odd = a.sfac[2][0], sz_even = a.sfac[2][1], sz_bnd = a.sfac[2][2];
 
     // Register rings: slot of absolute plane p = (p - zb + RAD) mod S.
     number       ring[NS][R][S];
@@ -354,23 +419,27 @@ namespace ppf
     number       ring_v[S][R][NF][PPF_N_SLOTS];
     unsigned int ring_fb[S]; // face bits of the plane in the slot (uniform)
 
+    // Source plane of z (wrap / mirror only outside [0, n2)).
+    auto source_z = [&](int z) {
+      return unsigned(z) < unsigned(n2) ? z : source_coord(z, n2, p2);
+    };
     auto load_own = [&](int z, int slot) {
-      const int     zs = source_coord(z, n2, p2);
-      const int64_t po = int64_t(zs) * plane_size;
-      ring_fb[slot]    = face_bits(zs, n2, p2, 2);
+      const int          zs = source_z(z);
+      const unsigned int po = unsigned(zs) * plane_bytes;
+      ring_fb[slot]         = face_bits(zs, n2, p2, 2);
 #pragma unroll
       for (int s = 0; s < NS; ++s)
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          ring[s][r][slot] = a.op[s][po + own_off[r]];
+          ring[s][r][slot] = buffer_load<number>(r_op[s], own_b[r], po);
     };
     auto load_inputs = [&](int z, int slot) {
-      const int64_t po = int64_t(source_coord(z, n2, p2)) * plane_size;
+      const unsigned int po = unsigned(source_z(z)) * plane_bytes;
 #pragma unroll
       for (int s = 0; s < NS; ++s)
         {
-          ring_hy[s][slot] = a.op[s][po + hy_off];
-          ring_hx[s][slot] = a.op[s][po + hx_off];
+          ring_hy[s][slot] = buffer_load<number>(r_op[s], hy_b, po);
+          ring_hx[s][slot] = buffer_load<number>(r_op[s], hx_b, po);
         }
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -378,7 +447,7 @@ namespace ppf
         for (int f = 0; f < NF; ++f)
 #pragma unroll
           for (int s = 0; s < PPF_N_SLOTS; ++s)
-            ring_v[slot][r][f][s] = a.val[f][s][po + own_off[r]];
+            ring_v[slot][r][f][s] = buffer_load<number>(r_val[f][s], own_b[r], po);
     };
 
     // Zeroes the operands of ring slot k on their Dirichlet faces.
@@ -392,7 +461,8 @@ namespace ppf
     };
 
     // One plane.  MASKED: face tests (constrained reads, Dirichlet outputs); without
-    // it the tile's read range contains no boundary node of a masked face.
+    // it the tile's read range contains no boundary node of a masked face.  Chunks
+    // start at even z and the ring length S is even, so plane z has the parity of PH.
     auto step = [&]<int PH, bool MASKED>(std::integral_constant<int, PH>,
                                           std::integral_constant<bool, MASKED>, int z) {
       constexpr int cs  = (PH + RAD) % S; // centre slot
@@ -430,54 +500,46 @@ namespace ppf
               hyv = ((fb_hy | ring_fb[cs]) & dm) ? number(0) : hyv;
               hxv = ((fb_hx | ring_fb[cs]) & dm) ? number(0) : hxv;
             }
-          if (hy_on)
-            lds[buf][s][hy_row][RAD + tx] = hyv;
-          if (hx_on)
-            lds[buf][s][RAD + hx_row][hx_col] = hxv;
+          lds[buf][s][hy_ly][RAD + tx] = hyv;
+          lds[buf][s][hx_ly][hx_lx]    = hxv;
         }
 
-      // z taps and scale factor (uniform).
+      // z taps (compile-time parity) and scale factor (uniform).
       const unsigned int fbz  = ring_fb[cs];
-      const int64_t      zoff = int64_t(z) * plane_size;
+      const unsigned int zoff = unsigned(z) * plane_bytes;
       number             wz0 = number(0), wz1 = number(0), wz2 = number(0), sz = number(1);
       if constexpr (dim == 3)
         {
-          const bool pz = degree == 2 && (z & 1);
-          wz0           = pz ? a.tap[2][1][0] : a.tap[2][0][0];
-          wz1           = pz ? a.tap[2][1][1] : a.tap[2][0][1];
-          wz2           = pz ? a.tap[2][1][2] : a.tap[2][0][2];
-          const int c  = scale_class<degree>(z, n2, p2);
-          sz           = c == 0 ? a.sfac[2][0] : (c == 1 ? a.sfac[2][1] : a.sfac[2][2]);
+          constexpr bool pz = degree == 2 && (PH & 1);
+          wz0               = pz ? a.tap[2][1][0] : a.tap[2][0][0];
+          wz1               = pz ? a.tap[2][1][1] : a.tap[2][0][1];
+          wz2               = pz ? a.tap[2][1][2] : a.tap[2][0][2];
+          sz = (!p2 && (z == 0 || z == n2 - 1)) ? sz_bnd : (pz ? sz_odd : sz_even);
         }
       __syncthreads();
 
 #pragma unroll
       for (int r = 0; r < R; ++r)
         {
-          const bool   py  = degree == 2 && (ys[r] & 1);
-          const int    cy  = scale_class<degree>(ys[r], n1, p1);
-          const number wy1 = py ? a.tap[1][1][1] : a.tap[1][0][1];
-          const number wy2 = py ? a.tap[1][1][2] : a.tap[1][0][2];
-          const number c0  = wx0 + ((py ? a.tap[1][1][0] : a.tap[1][0][0]) + wz0);
-          const number syz =
-            (cy == 0 ? a.sfac[1][0] : (cy == 1 ? a.sfac[1][1] : a.sfac[1][2])) * sz;
-          const int ly = RAD + w + stencil_waves * r;
-          const int lx = RAD + tx;
+          const number c0  = wx0 + (wy0[r] + wz0);
+          const number syz = sy[r] * sz;
+          const int    ly  = RAD + w + stencil_waves * r;
+          const int    lx  = RAD + tx;
 
           number lap[NS];
 #pragma unroll
           for (int s = 0; s < NS; ++s)
             {
-              const number(&L)[LY][LX] = lds[buf][s];
+              const number(&L)[LY + 1][LX] = lds[buf][s];
               number acc = c0 * ctr[s][r];
               acc += wx1 * (L[ly][lx - 1] + L[ly][lx + 1]);
-              acc += wy1 * (L[ly - 1][lx] + L[ly + 1][lx]);
+              acc += wy1[r] * (L[ly - 1][lx] + L[ly + 1][lx]);
               if constexpr (dim == 3)
                 acc += wz1 * (ring[s][r][(cs + S - 1) % S] + ring[s][r][(cs + 1) % S]);
               if constexpr (RAD == 2)
                 {
                   acc += wx2 * (L[ly][lx - 2] + L[ly][lx + 2]);
-                  acc += wy2 * (L[ly - 2][lx] + L[ly + 2][lx]);
+                  acc += wy2[r] * (L[ly - 2][lx] + L[ly + 2][lx]);
                   if constexpr (dim == 3)
                     acc += wz2 * (ring[s][r][(cs + S - 2) % S] + ring[s][r][(cs + 2) % S]);
                 }
@@ -486,8 +548,7 @@ namespace ppf
 
           // Hook at the node; lanes and rows without an output node evaluate it on
           // clamped values and store nothing.
-          const unsigned int fb    = fbx | fby[r] | fbz;
-          const int64_t      node  = zoff + own_off[r];
+          const unsigned int fb = fbx | fby[r] | fbz;
           prismspf::FieldContainer<dim, degree, number> fc;
           fc.q_point[0] = number(a.q_lower[0] + xs * a.q_step[0]);
           fc.q_point[1] = number(a.q_lower[1] + ys[r] * a.q_step[1]);
@@ -537,10 +598,10 @@ namespace ppf
                   // dirichlet_faces_kernel, which writes the constraint values.
                   const bool dir = (fb & out.dirichlet_mask[f]) != 0;
                   if (valid && !(dir && out.dirichlet_write == dirichlet_value))
-                    out.dst[f][node] = dir ? number(0) : scale * acc;
+                    buffer_store(dir ? number(0) : scale * acc, r_dst[f], own_b[r], zoff);
                 }
               else if (valid)
-                out.dst[f][node] = scale * acc;
+                buffer_store(scale * acc, r_dst[f], own_b[r], zoff);
             }
         }
     };
@@ -595,8 +656,11 @@ namespace ppf
   template <typename number>
   __global__ void __launch_bounds__(256)
   dirichlet_faces_kernel(number *__restrict__ dst, const number *__restrict__ values, int n0,
-                         int n1, int n2, int p0, int p1, int p2, unsigned int mask)
+                         int n1, int n2, int p0, int p1, int p2, unsigned int mask,
+                         const int *halt)
   {
+    if (halted(halt))
+      return;
     const int face   = blockIdx.y;
     const int axis   = face >> 1;
     const int n[3]   = {n0, n1, n2};

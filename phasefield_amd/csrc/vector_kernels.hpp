@@ -194,8 +194,14 @@ namespace ppf
     int           n;
   };
 
+  /// Probe of entry 0 of every field.  Writes (tag << 5) | (bad + 1) to *result, where
+  /// bad is the first field whose entry 0 is not finite (-1: none), in one store so that
+  /// a host polling the word sees the tag and the result together.  When a field is bad
+  /// and halt is given, sets *halt so that the launches carrying the flag that follow on
+  /// the stream do nothing.
   __global__ void
-  finite_probe_fields_kernel(int *__restrict__ first_bad, const ProbeFields fields)
+  finite_probe_fields_kernel(int *__restrict__ result, const ProbeFields fields,
+                             int *__restrict__ halt, int tag)
   {
     if (threadIdx.x == 0 && blockIdx.x == 0)
       {
@@ -203,7 +209,9 @@ namespace ppf
         for (int f = 0; f < fields.n && f < PPF_MAX_FIELDS; ++f)
           if (bad < 0 && !isfinite(fields.ptr[f][0]))
             bad = f;
-        *first_bad = bad;
+        if (bad >= 0 && halt)
+          *halt = 1;
+        *reinterpret_cast<volatile int *>(result) = (tag << 5) | (bad + 1);
       }
   }
 
