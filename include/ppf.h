@@ -97,7 +97,8 @@ int ppf_get_info(ppf_ctx *ctx, ppf_info *info);
 int ppf_field_name(ppf_ctx *ctx, int field, char *buf, int len);
 
 /* Options: "operator_path" = auto|generic|split|split_node (split: tiled stencil
- * kernel, split_node: one thread per node); "stencil_rows" = 0|2|4 and
+ * kernel, split_node: one thread per node); "stencil_waves" = 0|4|8,
+ * "stencil_rows" = 0|1|2|4 (waves x rows: 4x2, 4x4, 8x1 or 8x2) and
  * "stencil_z_chunk" = <planes> (0: automatic) tune the tiled kernel's workgroup
  * shape; "synchronize" = 1|0 (0 leaves ppf_apply_operator asynchronous on the
  * context's stream);

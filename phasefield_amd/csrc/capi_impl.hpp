@@ -224,6 +224,8 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
       }
     else if (k == "stencil_rows")
       c.rt.stencil_rows = std::stoi(v);
+    else if (k == "stencil_waves")
+      c.rt.stencil_waves = std::stoi(v);
     else if (k == "stencil_z_chunk")
       c.rt.stencil_z_chunk = std::stoi(v);
     else if (k == "stencil_target_blocks")

@@ -693,14 +693,24 @@ namespace ppf
       if (int64_t(a.n[0]) * a.n[1] * a.n[2] * int64_t(sizeof(number)) > (int64_t(1) << 32))
         return false;
 
-      int rows = stencil_rows;
+      // Tile shape: waves per workgroup x rows per wave.  Default 4 x 2 (measured
+      // faster than 4 x 4 at 129^3 and 257^3: fewer registers); two staged operands
+      // take 4 x 2 or 8 x 1 (more rows would spill in FP64).
+      int rows = stencil_rows, waves = stencil_waves;
       if (rows == 0)
-        rows = 2; // measured faster than 4 at 129^3 and 257^3 (fewer registers)
-      if (rows != 2 && rows != 4)
-        throw Error(status_invalid, "stencil_rows must be 2 or 4");
-      if (a.n_ops > 1)
-        rows = 2;
-      const int ty = stencil_waves * rows;
+        rows = waves == 8 ? 1 : 2;
+      if (waves == 0)
+        waves = 4;
+      const bool shape_ok = (waves == 4 && (rows == 2 || rows == 4)) ||
+                            (waves == 8 && (rows == 1 || rows == 2));
+      if (!shape_ok)
+        throw Error(status_invalid, "stencil tile: waves x rows must be 4x2, 4x4, 8x1 or 8x2");
+      if (a.n_ops > 1 && rows * waves > 8)
+        {
+          waves = 4;
+          rows  = 2;
+        }
+      const int ty = waves * rows;
       // Full tiles only where a few columns or rows would otherwise occupy a whole tile
       // row (e.g. 257 = 4 x 64 + 1): those are left to the remainder workgroups.
       int       tx_n = a.n[0], ty_n = a.n[1];
@@ -737,36 +747,49 @@ namespace ppf
           a.chunks_z = 1;
         }
       const int64_t nblocks = int64_t(a.tiles_x) * a.tiles_y * a.chunks_z;
-      const int64_t nrem    = (a.rem_a + a.rem_b + stencil_threads - 1) / stencil_threads;
+      const int     wg_threads = stencil_tx * waves;
+      const int64_t nrem       = (a.rem_a + a.rem_b + wg_threads - 1) / wg_threads;
       if (nblocks + nrem >= (int64_t(1) << 31))
         return false;
       a.n_blocks = int(nblocks);
 
       const auto &pde = *pde_of<number>();
-      const dim3  grid(unsigned(nblocks + nrem)), threads(stencil_tx, stencil_waves);
-      auto        go = [&](auto rows_c, auto ops_c) {
+      const dim3  grid(unsigned(nblocks + nrem)), threads(stencil_tx, unsigned(waves));
+      auto        go = [&](auto waves_c, auto rows_c, auto ops_c) {
+        constexpr int WV = decltype(waves_c)::value;
         constexpr int RR = decltype(rows_c)::value;
         constexpr int NS = decltype(ops_c)::value;
         if (lhs)
           hipLaunchKernelGGL(
-            (stencil_operator_kernel<PDE<number>, dim, degree, number, true, id, App::n_fields, RR, NS>),
+            (stencil_operator_kernel<PDE<number>, dim, degree, number, true, id, App::n_fields, WV, RR, NS>),
             grid, threads, 0, stream, pde, a, out, timer);
         else
           hipLaunchKernelGGL(
-            (stencil_operator_kernel<PDE<number>, dim, degree, number, false, id, App::n_fields, RR, NS>),
+            (stencil_operator_kernel<PDE<number>, dim, degree, number, false, id, App::n_fields, WV, RR, NS>),
             grid, threads, 0, stream, pde, a, out, timer);
       };
+      using I1 = std::integral_constant<int, 1>;
       using I2 = std::integral_constant<int, 2>;
       using I4 = std::integral_constant<int, 4>;
-      using S1 = std::integral_constant<int, 1>;
-      using S2 = std::integral_constant<int, 2>;
-      // Two staged operands use 2 rows per wave (4 would spill registers in FP64).
+      using I8 = std::integral_constant<int, 8>;
       if (a.n_ops > 1)
-        go(I2 {}, S2 {});
+        {
+          if (waves == 8)
+            go(I8 {}, I1 {}, I2 {});
+          else
+            go(I4 {}, I2 {}, I2 {});
+        }
+      else if (waves == 8)
+        {
+          if (rows == 1)
+            go(I8 {}, I1 {}, I1 {});
+          else
+            go(I8 {}, I2 {}, I1 {});
+        }
       else if (rows == 2)
-        go(I2 {}, S1 {});
+        go(I4 {}, I2 {}, I1 {});
       else
-        go(I4 {}, S1 {});
+        go(I4 {}, I4 {}, I1 {});
       if (out.dirichlet_write == dirichlet_value)
         for (int f = 0; f < PPF_MAX_FIELDS; ++f)
           if ((out.out_mask & (1u << f)) && out.dirichlet_mask[f])
@@ -1853,9 +1876,11 @@ namespace ppf
     GridDesc                               fine {};
     hipStream_t                            stream = nullptr;
     OperatorPath                           path   = OperatorPath::Auto;
-    /// Tile rows per wave of the stencil kernel (0: chosen from the lattice size) and
-    /// z-planes per workgroup (0: chosen to give about `stencil_target_blocks` groups).
+    /// Tile shape of the stencil kernel: rows per wave and waves per workgroup (0:
+    /// defaults), and z-planes per workgroup (0: chosen to give about
+    /// `stencil_target_blocks` groups).
     int                                    stencil_rows    = 0;
+    int                                    stencil_waves   = 0;
     int                                    stencil_z_chunk = 0;
     int                                    stencil_target_blocks = 2048;
 

@@ -22,9 +22,9 @@
 // (Q2: (1,-8,14,-8,1) -> (14,-16,2); Q1: (-1,2,-1) -> (2,-2)), so taps depend only on
 // the parity of the coordinate and boundaries are handled by the source index.
 //
-// Work decomposition (3-D): a workgroup of 4 wavefronts owns a tile of 64 x (4R) nodes
-// in the x-y plane and marches over a chunk of z-planes.  Lane = x; wave w owns rows
-// w, w+4, ..., w+4(R-1) (wave-uniform y).
+// Work decomposition (3-D): a workgroup of W wavefronts (4 or 8) owns a tile of
+// 64 x (W R) nodes in the x-y plane and marches over a chunk of z-planes.  Lane = x;
+// wave w owns rows w, w+W, ..., w+W(R-1) (wave-uniform y).
 //   * Every stencil operand value, halo value and hook value input is loaded
 //     D = p+1 planes before the plane that consumes it, into a register ring whose
 //     slot indices are compile-time constants (the z-loop is unrolled by the ring
@@ -48,8 +48,6 @@
 namespace ppf
 {
   constexpr int stencil_tx      = 64; // tile width in x: one wavefront
-  constexpr int stencil_waves   = 4;  // wavefronts per workgroup
-  constexpr int stencil_threads = stencil_tx * stencil_waves;
   constexpr int stencil_max_ops = 2;
 
   template <typename number>
@@ -274,8 +272,8 @@ namespace ppf
   }
 
   template <class PDE, int dim, int degree, typename number, bool lhs, int block_id, int NF,
-            int R, int NS>
-  __global__ void __launch_bounds__(stencil_threads)
+            int WV, int R, int NS>
+  __global__ void __launch_bounds__(stencil_tx * WV)
   stencil_operator_kernel(const PDE                       pde,
                           const StencilArgs<number>       a,
                           const OutTable<number>          out,
@@ -283,11 +281,12 @@ namespace ppf
   {
     constexpr int RAD = degree;
     constexpr int T   = 2 * RAD + 1;
-    constexpr int TY  = stencil_waves * R;
+    constexpr int TY  = WV * R;
     constexpr int LX  = stencil_tx + 2 * RAD;
     constexpr int LY  = TY + 2 * RAD;
     constexpr int D   = RAD + 1; // load distance in planes
     constexpr int S   = T + 1;   // ring length
+    static_assert(WV >= 2 * RAD, "the y-halo rows are loaded by one wave each");
     __shared__ number lds[2][NS][LY + 1][LX]; // row LY: spare target of idle lanes
 
     const int tx  = threadIdx.x;
@@ -301,7 +300,7 @@ namespace ppf
     if (int(blockIdx.x) >= a.n_blocks)
       {
         // Remainder workgroup: one node per thread.
-        int64_t t = int64_t(int(blockIdx.x) - a.n_blocks) * stencil_threads +
+        int64_t t = int64_t(int(blockIdx.x) - a.n_blocks) * (stencil_tx * WV) +
                     threadIdx.y * stencil_tx + threadIdx.x;
         int x, y, z;
         if (t < a.rem_a)
@@ -374,7 +373,7 @@ namespace ppf
 #pragma unroll
     for (int r = 0; r < R; ++r)
       {
-        const int y = y0 + w + stencil_waves * r;
+        const int y = y0 + w + WV * r;
         yout[r]     = y < a.tile_n[1];
         ys[r]       = source_coord(y, n1, p1);
         fby[r]      = face_bits(ys[r], n1, p1, 1);
@@ -390,7 +389,7 @@ namespace ppf
     // Halo assignments: wave w < 2 RAD loads one y-halo row of the tile's columns;
     // threads tid < 2 RAD TY load one x-halo node (the others repeat the y-halo load).
     // Lanes without a halo node store to the spare LDS row, so staging has no branch.
-    __builtin_assume(w >= 0 && w < stencil_waves);
+    __builtin_assume(w >= 0 && w < WV);
     const int          hy_row = w < RAD ? w : TY + w; // LDS row
     const bool         hy_on  = w < 2 * RAD;
     const int          hy_y   = source_coord(y0 - RAD + hy_row, n1, p1);
@@ -491,7 +490,7 @@ namespace ppf
           for (int r = 0; r < R; ++r)
             {
               ctr[s][r] = ring[s][r][cs];
-              lds[buf][s][RAD + w + stencil_waves * r][RAD + tx] = ctr[s][r];
+              lds[buf][s][RAD + w + WV * r][RAD + tx] = ctr[s][r];
             }
           number hyv = ring_hy[s][cs], hxv = ring_hx[s][cs];
           if constexpr (MASKED)
@@ -523,7 +522,7 @@ namespace ppf
         {
           const number c0  = wx0 + (wy0[r] + wz0);
           const number syz = sy[r] * sz;
-          const int    ly  = RAD + w + stencil_waves * r;
+          const int    ly  = RAD + w + WV * r;
           const int    lx  = RAD + tx;
 
           number lap[NS];

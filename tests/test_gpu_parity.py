@@ -95,8 +95,10 @@ def test_operator_matches_oracle(app, text, ops, path):
             assert err <= 1e-12, (block_id, which, scaling, err)
 
 
-# Tile shapes of the stencil kernel: (rows per wave, z-planes per workgroup; 0 = automatic).
-TILINGS = [(2, 0), (4, 0), (2, 5), (4, 7), (2, 64), (4, 1)]
+# Tile shapes of the stencil kernel: (waves per workgroup, rows per wave, z-planes per
+# workgroup; 0 = automatic; Q2 rounds odd chunks up to even).
+TILINGS = [(4, 2, 0), (4, 4, 0), (4, 2, 5), (4, 4, 7), (4, 2, 64), (4, 4, 1),
+           (8, 1, 0), (8, 2, 0), (8, 1, 3), (8, 2, 6)]
 TILING_CASES = [
     # 257 x 33 x 33 nodes, Dirichlet on every face: tiles in the interior run the
     # variant without face tests, the others the masked variant.
@@ -107,6 +109,9 @@ TILING_CASES = [
      [(0, OP_RHS, 1), (1, OP_RHS, 1)]),
     ("allen_cahn_2d_q1", prm.allen_cahn(dim=2, size=30.0, refinement=5, subdivisions=[5, 3]),
      [(1, OP_RHS, 1)]),
+    # Blocks with two staged operands (4 x 2 and 8 x 1 tiles only).
+    ("precipitate_3d_q1", prm.precipitate(size=8.0, refinement=2, subdivisions=(1, 1, 1)),
+     [(0, OP_RHS, 1), (1, OP_RHS, 1), (2, OP_RHS, 1)]),
 ]
 
 
@@ -116,7 +121,8 @@ def test_stencil_tilings_match_oracle(app, text, ops):
     info = ctx.info
     rng = np.random.default_rng(21)
     seed_fields(ctx, orc, rng)
-    for rows, zc in TILINGS:
+    for waves, rows, zc in TILINGS:
+        ctx.set_option("stencil_waves", waves)
         ctx.set_option("stencil_rows", rows)
         ctx.set_option("stencil_z_chunk", zc)
         for block_id, which, nb in ops:
@@ -129,7 +135,7 @@ def test_stencil_tilings_match_oracle(app, text, ops):
                 ctx.apply_operator(block_id, which, scaling, FP64, d_src.data_ptr(),
                                    d_dst.data_ptr())
                 err = rel_err(d_dst.cpu().numpy(), block_to_stored(ref, info, nb))
-                assert err <= 1e-12, (rows, zc, block_id, which, scaling, err)
+                assert err <= 1e-12, (waves, rows, zc, block_id, which, scaling, err)
 
 
 @pytest.mark.parametrize("app,text,ops", OPERATOR_CASES[2:6], ids=[c[0] for c in OPERATOR_CASES[2:6]])
