@@ -158,7 +158,7 @@ namespace ppf
       PPF_HIP_CHECK(hipStreamCreate(&stream));
       reduce_buf.resize(2 * reduce_blocks + 8);
       scalar_buf.resize(8);
-      halt_flag.resize(1);
+      halt_flag.resize(2);
       setup_storage();
       compute_dirichlet_values(fine, dirichlet_values);
     }
@@ -556,7 +556,7 @@ namespace ppf
       dispatch_block_id(typename App::BlockIds {}, blk.id, [&](auto bid) {
         constexpr int id = decltype(bid)::value;
         if (use_split && path != OperatorPath::SplitNode &&
-            launch_stencil<number, id>(lhs, g, deps, out, plan))
+            launch_stencil<number, id>(b, lhs, g, deps, out, plan))
           return;
         if (use_split)
           {
@@ -593,7 +593,7 @@ namespace ppf
     /// has more stencil operands than the kernel stages (the per-node kernel runs it).
     template <typename number, int id>
     bool
-    launch_stencil(bool lhs, const GridDesc &g, const DepTable<number> &deps,
+    launch_stencil(int b, bool lhs, const GridDesc &g, const DepTable<number> &deps,
                    const OutTable<number> &out, const SplitPlan &plan)
     {
       StencilArgs<number> a {};
@@ -754,6 +754,24 @@ namespace ppf
       a.n_blocks = int(nblocks);
 
       const auto &pde = *pde_of<number>();
+      // The increment's NaN probe rides on this launch if it is the block's last kernel
+      // (no Dirichlet-value faces kernel follows) and node 0 lies in tile 0.
+      bool faces_follow = false;
+      if (out.dirichlet_write == dirichlet_value)
+        for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+          faces_follow = faces_follow || ((out.out_mask & (1u << f)) && out.dirichlet_mask[f]);
+      if constexpr (std::is_same_v<number, double>)
+        if (probe_req.pending && probe_req.block == b && !faces_follow && a.n_blocks > 0)
+          {
+            a.probe_result = probe_req.result;
+            a.probe_halt   = probe_req.halt;
+            a.probe_tag    = probe_req.tag;
+            a.probe_n      = probe_req.fields.n;
+            for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+              a.probe_ptr[f] = probe_req.fields.ptr[f];
+            probe_req.pending = false;
+            probe_req.taken   = true;
+          }
       const dim3  grid(unsigned(nblocks + nrem)), threads(stencil_tx, unsigned(waves));
       auto        go = [&](auto waves_c, auto rows_c, auto ops_c) {
         constexpr int WV = decltype(waves_c)::value;
@@ -1650,23 +1668,28 @@ namespace ppf
             storage[f][age].swap(storage[f][age - 1]);
     }
 
-    /// Enqueues the solve blocks of the current increment, then the NaN probe of
-    /// entry 0 of every field, whose result goes to pinned slot `slot`; no host wait.
-    /// With `halt` given, every launch carries the flag (and does nothing once an
-    /// earlier probe has set it) and the probe sets it on a non-finite entry.
+    /// Enqueues the solve blocks of the current increment and its NaN probe of entry 0
+    /// of every field, whose result goes to pinned slot `slot`; no host wait.  The probe
+    /// rides on the last operator launch when that launch is the stencil kernel of an
+    /// explicit block, else it is a kernel of its own.
+    ///
+    /// With `halt` (two words) given, increment k's launches read halt[(k + 1) & 1],
+    /// which the probe of increment k - 1 sets on a non-finite entry, and do nothing
+    /// once it is set; increment k's probe sets halt[k & 1].  A probe that shares a
+    /// launch with its increment thus never stops that increment's own workgroups.
     void
     enqueue_increment(bool do_output, int slot, int *halt)
     {
-      enqueue_halt = halt;
+      const unsigned int inc = timer.get_increment();
+      std::vector<int>   solve;
       for (std::size_t b = 0; b < blocks.size(); ++b)
         {
           const auto timing = blocks[b].solve_timing;
           if ((timing == prismspf::PostProcess || timing == prismspf::NucleationRate) &&
               !do_output)
             continue;
-          solve_block(int(b));
+          solve.push_back(int(b));
         }
-      enqueue_halt = nullptr;
       ProbeFields pf {};
       pf.n = int(fields.size());
       for (std::size_t f = 0; f < fields.size(); ++f)
@@ -1674,9 +1697,26 @@ namespace ppf
       probe_tag[slot] = int(++probe_count & 0x3ffffffu);
       int *result     = probe_flag.get(2) + slot;
       *reinterpret_cast<volatile int *>(result) = -1;
-      hipLaunchKernelGGL(finite_probe_fields_kernel, dim3(1), dim3(1), 0, stream, result, pf,
-                         halt, probe_tag[slot]);
-      PPF_HIP_CHECK(hipGetLastError());
+      int *halt_out = halt ? halt + (inc & 1u) : nullptr;
+      probe_req         = {};
+      probe_req.result  = result;
+      probe_req.halt    = halt_out;
+      probe_req.tag     = probe_tag[slot];
+      probe_req.fields  = pf;
+      probe_req.block   = solve.empty() ? -1 : solve.back();
+      probe_req.pending = !solve.empty() && inc > 0 &&
+                          blocks[solve.back()].solve_type == prismspf::Explicit;
+      enqueue_halt = halt ? halt + ((inc + 1u) & 1u) : nullptr;
+      for (int b : solve)
+        solve_block(b);
+      enqueue_halt = nullptr;
+      if (!probe_req.taken)
+        {
+          hipLaunchKernelGGL(finite_probe_fields_kernel, dim3(1), dim3(1), 0, stream, result,
+                             pf, halt_out, probe_tag[slot]);
+          PPF_HIP_CHECK(hipGetLastError());
+        }
+      probe_req = {};
     }
 
     /// Waits for the probe result in `slot` by polling its pinned word (no event or
@@ -1768,7 +1808,7 @@ namespace ppf
     {
       int  status = 0;
       int *halt   = halt_flag.get();
-      hipLaunchKernelGGL(fill_kernel<int>, dim3(1), dim3(64), 0, stream, halt, 0, int64_t(1));
+      hipLaunchKernelGGL(fill_kernel<int>, dim3(1), dim3(64), 0, stream, halt, 0, int64_t(2));
       PPF_HIP_CHECK(hipGetLastError());
       bool      pending = false; // an increment enqueued whose probe is unread
       int       pending_slot = 0;
@@ -1897,6 +1937,17 @@ namespace ppf
     PinnedBuffer<int>                              probe_flag; // two probe slots
     int                                            probe_tag[2] = {0, 0};
     unsigned int                                   probe_count  = 0;
+    /// The NaN probe of the increment being enqueued, until a launch takes it.
+    struct ProbeRequest
+    {
+      int        *result  = nullptr;
+      int        *halt    = nullptr;
+      int         tag     = 0;
+      ProbeFields fields  = {};
+      int         block   = -1;
+      bool        pending = false;
+      bool        taken   = false;
+    } probe_req;
     DeviceBuffer<int>                              halt_flag;
     const int                                     *enqueue_halt = nullptr;
     std::string                                    log;

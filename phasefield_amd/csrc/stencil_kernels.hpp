@@ -86,7 +86,31 @@ namespace ppf
     signed char   val_alias[PPF_MAX_FIELDS][PPF_N_SLOTS];
     /// Union of all face masks the operator reads or writes (0: no face test at all).
     unsigned int  any_dmask;
+    /// Entry-0 NaN probe of an increment, fused into its last launch (probe_result
+    /// null: none).  The thread that owns node 0 writes (tag << 5) | (bad + 1) to
+    /// *probe_result once its own outputs are stored, bad being the first field whose
+    /// entry 0 is not finite (-1: none), and sets *probe_halt on a bad field.
+    int          *probe_result;
+    int          *probe_halt;
+    int           probe_tag;
+    int           probe_n;
+    const double *probe_ptr[PPF_MAX_FIELDS];
   };
+
+  template <typename number>
+  __device__ void
+  stencil_probe(const StencilArgs<number> &a)
+  {
+    // Orders this thread's stores of node 0 before the reads below.
+    __threadfence();
+    int bad = -1;
+    for (int f = 0; f < a.probe_n && f < PPF_MAX_FIELDS; ++f)
+      if (bad < 0 && !isfinite(*reinterpret_cast<const volatile double *>(a.probe_ptr[f])))
+        bad = f;
+    if (bad >= 0 && a.probe_halt)
+      *reinterpret_cast<volatile int *>(a.probe_halt) = 1;
+    *reinterpret_cast<volatile int *>(a.probe_result) = (a.probe_tag << 5) | (bad + 1);
+  }
 
   /// Source node along one axis for lattice coordinate i, which may lie up to a stencil
   /// radius outside [0, n): periodic axes wrap, other axes reflect at the boundary node
@@ -602,6 +626,11 @@ namespace ppf
               else if (valid)
                 buffer_store(scale * acc, r_dst[f], own_b[r], zoff);
             }
+          // Node 0 (lane 0, wave 0, row 0, plane 0 of workgroup 0, which owns tile 0 of
+          // chunk 0) closes the increment's NaN probe.
+          if constexpr (PH == 0)
+            if (a.probe_result && r == 0 && z == 0 && blockIdx.x == 0 && w == 0 && tx == 0)
+              stencil_probe(a);
         }
     };
 
