@@ -255,6 +255,12 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
         for (auto &b : c.rt.blocks)
           b.linear_solver_parameters.max_iterations = unsigned(std::stoul(v));
       }
+    else if (k == "linear_initial_guess")
+      {
+        if (v != "previous" && v != "zero")
+          throw ppf::Error(ppf::status_invalid, "linear_initial_guess must be previous|zero");
+        c.rt.zero_initial_guess = v == "zero";
+      }
     else
       throw ppf::Error(ppf::status_invalid, "unknown option " + k);
   });
@@ -339,21 +345,11 @@ ppf_apply_operator(ppf_ctx *ctx, int block_id, int which, int scaling, int preci
     auto run = [&](auto tag) {
       using T    = decltype(tag);
       auto lf    = typename ppf::Runtime<PPF_APP>::template LevelFields<T> {};
-      lf.grid    = &rt.fine;
       std::vector<ppf::DeviceBuffer<T>> shadow;
       if constexpr (std::is_same_v<T, double>)
         lf = rt.fine_fields();
       else
-        {
-          // FP32 copies of the non-operand fields.
-          for (std::size_t f = 0; f < rt.storage.size(); ++f)
-            for (std::size_t a = 0; a < rt.storage[f].size(); ++a)
-              {
-                shadow.emplace_back(rt.fine.n_nodes);
-                rt.convert(shadow.back().get(), rt.storage[f][a].get(), rt.fine.n_nodes);
-                lf.ptr[f][a + 1] = shadow.back().get();
-              }
-        }
+        lf = rt.fp32_fields(b, which != PPF_OP_RHS, shadow); // FP32 copies of what it reads
       std::vector<const T *> src;
       std::vector<T *>       dst;
       for (int k = 0; k < nbf; ++k)
@@ -384,16 +380,8 @@ ppf_inverse_diagonal(ppf_ctx *ctx, int block_id, int scaling, int precision, voi
     const int b  = rt.block_index(block_id);
     if (precision == PPF_FP32)
       {
-        auto lf = typename ppf::Runtime<PPF_APP>::template LevelFields<float> {};
-        lf.grid = &rt.fine;
         std::vector<ppf::DeviceBuffer<float>> shadow;
-        for (std::size_t f = 0; f < rt.storage.size(); ++f)
-          for (std::size_t a = 0; a < rt.storage[f].size(); ++a)
-            {
-              shadow.emplace_back(rt.fine.n_nodes);
-              rt.convert(shadow.back().get(), rt.storage[f][a].get(), rt.fine.n_nodes);
-              lf.ptr[f][a + 1] = shadow.back().get();
-            }
+        const auto lf = rt.fp32_fields(b, true, shadow);
         rt.template inverse_diagonal<float>(b, lf, scaling, static_cast<float *>(d_dst));
       }
     else

@@ -483,6 +483,31 @@ namespace ppf
       return lf;
     }
 
+    /// FP32 copies of the field states that the rhs or lhs of block `b` reads (its
+    /// declared non-src dependencies), for FP32 operator applications on the fine grid.
+    LevelFields<float>
+    fp32_fields(int b, bool lhs, std::vector<DeviceBuffer<float>> &shadow)
+    {
+      LevelFields<float> lf {};
+      lf.grid               = &fine;
+      const auto &blk       = blocks[b];
+      constexpr int src_slot = prismspf::dependency_slot(prismspf::SRC);
+      shadow.clear();
+      shadow.reserve(PPF_MAX_FIELDS * (PPF_MAX_AGE + 1));
+      for (const auto &in : declared_inputs(lhs ? blk.dependencies_lhs : blk.dependencies_rhs))
+        {
+          if (in.slot == src_slot || lf.ptr[in.field][in.slot])
+            continue;
+          const int age = in.slot - 1;
+          if (age < 0 || age >= int(storage[in.field].size()))
+            throw Error(status_internal, "dependency storage not available");
+          shadow.emplace_back(fine.n_nodes);
+          convert(shadow.back().get(), storage[in.field][age].get(), fine.n_nodes);
+          lf.ptr[in.field][in.slot] = shadow.back().get();
+        }
+      return lf;
+    }
+
     /// Applies the rhs or lhs operator of block `b`:
     ///   dst = scaling * (assembled hook terms) with the Dirichlet rule `dwrite`.
     /// `src[k]`/`dst[k]` belong to the k-th field of the block (ascending field index).
@@ -1093,6 +1118,8 @@ namespace ppf
       }
 
       prepare_preconditioner(b);
+      if (zero_initial_guess)
+        fill(ls.x_homog.get(), 0.0, ls.n);
       SolveStats st = cg(b, ls.x_homog.get(), ls.rhs.get(),
                          lin_params(b).tolerance * normalization(b), lin_params(b).max_iterations);
 
@@ -1207,17 +1234,10 @@ namespace ppf
                          T *r, T *p, T *ap, T *z)
     {
       // Start vector: splitmix64 of the block-vector index (field-major, stored node
-      // order), uniform in [-0.5, 0.5).
-      std::vector<T> host(n);
-      for (int64_t i = 0; i < n; ++i)
-        {
-          uint64_t x = uint64_t(i) + 0x9e3779b97f4a7c15ull;
-          x          = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
-          x          = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
-          x ^= x >> 31;
-          host[i] = T(double(x >> 11) * (1.0 / 9007199254740992.0) - 0.5);
-        }
-      PPF_HIP_CHECK(hipMemcpyAsync(z, host.data(), sizeof(T) * n, hipMemcpyHostToDevice, stream));
+      // order), uniform in [-0.5, 0.5), generated on the device.
+      hipLaunchKernelGGL(splitmix_fill_kernel<T>, dim3(std::min<int64_t>(grid_for(n, 256), 4096)),
+                         dim3(256), 0, stream, z, n);
+      PPF_HIP_CHECK(hipGetLastError());
       // CG for A e = 0 from the pseudo-random guess: the initial residual is A x0, which
       // also has zero constrained rows.  z = P^{-1} r.
       A(r, z);
@@ -1923,6 +1943,9 @@ namespace ppf
     int                                    stencil_waves   = 0;
     int                                    stencil_z_chunk = 0;
     int                                    stencil_target_blocks = 2048;
+    /// Start every linear solve from zero instead of the previous solution (which the
+    /// reference keeps as the next initial guess, linear_solver.h:330-336).
+    bool                                   zero_initial_guess    = false;
 
     std::vector<std::vector<DeviceBuffer<double>>> storage;
     std::vector<DeviceBuffer<double>>              dirichlet_values;

@@ -101,6 +101,23 @@ namespace ppf
       inv[i] = d[i] == T(0) ? T(1) : T(1) / d[i];
   }
 
+  /// Start vector of the eigenvalue estimate: splitmix64 of the index, uniform in
+  /// [-0.5, 0.5) (integer mixing and exact 53-bit scaling: the same values as on a CPU).
+  template <typename T>
+  __global__ void
+  splitmix_fill_kernel(T *__restrict__ x, int64_t n)
+  {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+      {
+        uint64_t z = uint64_t(i) + 0x9e3779b97f4a7c15ull;
+        z          = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z          = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        x[i] = T(double(z >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+      }
+  }
+
   /// Stage 1 of a dot product: one partial per block (grid-stride, fixed grid).
   template <typename T>
   __global__ void __launch_bounds__(reduce_threads)

@@ -253,6 +253,24 @@ def test_mixed_precision_gmg_reaches_fp64_tolerance(device):
     assert rel_err(ctx.get_field(0, AGE_LATEST).ravel(), orc.get_field(0, AGE_LATEST)) <= 1e-6
 
 
+def test_linear_initial_guess_option(device):
+    """A repeated solve starts from the previous solution (the reference's warm start,
+    0 iterations); with linear_initial_guess=zero it repeats the first solve exactly."""
+    text = prm.poisson(dim=3, refinement=3, mg_depth=3, tolerance=1e-9)
+    ctx = Context("poisson_3d_q2", text)
+    ctx.run(0)
+    first = ctx.last_solve_stats(1)
+    u1 = ctx.get_field(0, 0).copy()
+    warm = ctx.linear_solve(1)
+    assert first.iterations > 0 and warm.iterations == 0
+    ctx.set_option("linear_initial_guess", "zero")
+    cold = ctx.linear_solve(1)
+    assert cold.iterations == first.iterations
+    assert np.array_equal(ctx.get_field(0, 0), u1)
+    with pytest.raises(PPFError):
+        ctx.set_option("linear_initial_guess", "random")
+
+
 ADJ_TEXT = prm.allen_cahn(dim=3, size=8.0, refinement=3, steps=0, dt=0.01, kappa=1.5)
 
 
