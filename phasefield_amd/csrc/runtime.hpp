@@ -158,6 +158,8 @@ namespace ppf
       PPF_HIP_CHECK(hipStreamCreate(&stream));
       reduce_buf.resize(2 * reduce_blocks + 8);
       scalar_buf.resize(8);
+      coarse_slots.resize(8);
+      coarse_flag.resize(1);
       halt_flag.resize(2);
       setup_storage();
       compute_dirichlet_values(fine, dirichlet_values);
@@ -400,6 +402,17 @@ namespace ppf
       hipLaunchKernelGGL(dot_partial_kernel<T>, dim3(reduce_blocks), dim3(reduce_threads), 0,
                          stream, reduce_buf.get(), x, y, n);
       return finish_reduction(reduce_blocks);
+    }
+
+    /// Deterministic dot product into a device slot (no host wait).
+    template <typename T>
+    void
+    dot_to(const T *x, const T *y, int64_t n, double *slot)
+    {
+      hipLaunchKernelGGL(dot_partial_kernel<T>, dim3(reduce_blocks), dim3(reduce_threads), 0,
+                         stream, reduce_buf.get(), x, y, n);
+      hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(reduce_threads), 0, stream, slot,
+                         reduce_buf.get(), reduce_blocks);
     }
 
     double
@@ -717,6 +730,25 @@ namespace ppf
       // 32-bit byte offsets in the kernel: arrays up to 4 GiB.
       if (int64_t(a.n[0]) * a.n[1] * a.n[2] * int64_t(sizeof(number)) > (int64_t(1) << 32))
         return false;
+      // A pending coarse-grid solve takes this operator (src = p, dst = ap) and runs the
+      // whole CG in one workgroup instead of one launch per vector operation.
+      if constexpr (std::is_same_v<number, double>)
+        if (coarse_pending && lhs)
+          {
+            coarse_pending = false;
+            auto launch    = [&](auto ops_c) {
+              constexpr int NS = decltype(ops_c)::value;
+              hipLaunchKernelGGL((coarse_cg_kernel<PDE<double>, dim, degree, id, App::n_fields, NS>),
+                                 dim3(1), dim3(reduce_threads), 0, stream, *pde_of<double>(), a,
+                                 out, timer, coarse_req);
+            };
+            if (a.n_ops > 1)
+              launch(std::integral_constant<int, 2> {});
+            else
+              launch(std::integral_constant<int, 1> {});
+            PPF_HIP_CHECK(hipGetLastError());
+            return true;
+          }
 
       // Tile shape: waves per workgroup x rows per wave.  Default 4 x 2 (measured
       // faster than 4 x 4 at 129^3 and 257^3: fewer registers); two staged operands
@@ -1244,24 +1276,41 @@ namespace ppf
       hipLaunchKernelGGL(diag_mult_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, stream, z,
                          inv_diag, r, n);
       copy(p, z, n);
-      double              rz = dot(r, z, n);
-      std::vector<double> alphas, betas;
-      for (unsigned int it = 0; it < cp.eig_cg_n_iterations && rz > 0.0; ++it)
+      // The CG scalars stay on the device (rz_i in slot i, pAp_i in slot its + 1 + i);
+      // the host reads them once, after the last step.
+      const int its = int(cp.eig_cg_n_iterations);
+      eig_slots.resize(2 * its + 2);
+      double *rz  = eig_slots.get();
+      double *pap = rz + its + 1;
+      fill(rz, 0.0, 2 * its + 2);
+      dot_to(r, z, n, rz);
+      const unsigned int nb = grid_for(n, 256);
+      for (int it = 0; it < its; ++it)
         {
           A(ap, p);
-          const double pap = dot(p, ap, n);
-          if (!(pap > 0.0))
+          dot_to(p, ap, n, pap + it);
+          hipLaunchKernelGGL(eig_r_update_kernel<T>, dim3(nb), dim3(256), 0, stream, r, ap, rz,
+                             pap, it, n);
+          hipLaunchKernelGGL(diag_mult_kernel<T>, dim3(nb), dim3(256), 0, stream, z, inv_diag,
+                             r, n);
+          dot_to(r, z, n, rz + it + 1);
+          hipLaunchKernelGGL(eig_p_update_kernel<T>, dim3(nb), dim3(256), 0, stream, p, z, rz,
+                             pap, it, n);
+        }
+      PPF_HIP_CHECK(hipGetLastError());
+      std::vector<double> host(2 * its + 2);
+      PPF_HIP_CHECK(hipMemcpyAsync(host.data(), rz, sizeof(double) * host.size(),
+                                   hipMemcpyDeviceToHost, stream));
+      sync();
+      // The host loop's coefficients: stop at the first non-positive rz or pAp.
+      std::vector<double> alphas, betas;
+      for (int it = 0; it < its; ++it)
+        {
+          const double rzi = host[it], papi = host[its + 1 + it];
+          if (!(rzi > 0.0) || !(papi > 0.0))
             break;
-          const double alpha = rz / pap;
-          alphas.push_back(alpha);
-          axpby(r, T(-alpha), ap, T(1), n);
-          hipLaunchKernelGGL(diag_mult_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, stream, z,
-                             inv_diag, r, n);
-          const double rz_new = dot(r, z, n);
-          const double beta   = rz_new / rz;
-          betas.push_back(beta);
-          rz = rz_new;
-          axpby(p, T(1), z, T(beta), n);
+          alphas.push_back(rzi / papi);
+          betas.push_back(host[it + 1] / rzi);
         }
       // Lanczos tridiagonal from the CG coefficients.
       const int           m = int(alphas.size());
@@ -1566,29 +1615,62 @@ namespace ppf
       if constexpr (std::is_same_v<T, float>)
         convert(r, l.vf[1].get(), n);
       auto A = [&](double *dst, const double *src) { level_vmult<double>(b, l, dst, src); };
-      fill(x, 0.0, n);
-      const double r0 = std::sqrt(dot(r, r, n));
-      if (r0 > 0.0)
+      if (l.grid.n_nodes <= coarse_fused_max_nodes)
         {
-          hipLaunchKernelGGL(diag_mult_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0,
-                             stream, z, l.inv_diag_d.get(), r, n);
-          copy(p, z, n);
-          double rz = dot(r, z, n);
-          for (unsigned int it = 0; it < lp.coarse_cg_max_iterations; ++it)
+          // One workgroup runs the whole solve when the level operator is a stencil
+          // operator (the operator launch below is taken over); otherwise that launch
+          // is an ordinary application and the host-driven loop follows.
+          coarse_req     = CoarseCgArgs {x, r, p, ap, z, l.inv_diag_d.get(), n,
+                                     lp.coarse_cg_tolerance, int(lp.coarse_cg_max_iterations)};
+          coarse_pending = true;
+          A(ap, p);
+          const bool fused = !coarse_pending;
+          coarse_pending   = false;
+          if (fused)
             {
-              A(ap, p);
-              const double alpha = rz / dot(p, ap, n);
-              axpby(x, alpha, p, 1.0, n);
-              axpby(r, -alpha, ap, 1.0, n);
-              if (std::sqrt(dot(r, r, n)) <= lp.coarse_cg_tolerance * r0)
-                break;
-              hipLaunchKernelGGL(diag_mult_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0,
-                                 stream, z, l.inv_diag_d.get(), r, n);
-              const double rz_new = dot(r, z, n);
-              axpby(p, 1.0, z, rz_new / rz, n);
-              rz = rz_new;
+              if constexpr (std::is_same_v<T, float>)
+                convert(l.vf[0].get(), x, n);
+              return;
             }
         }
+      fill(x, 0.0, n);
+      // Scalars and the stop flag stay on the device; the host looks at the flag every
+      // few steps (steps after convergence change nothing: their updates are skipped).
+      double *cs   = coarse_slots.get();
+      int    *done = coarse_flag.get();
+      dot_to(r, r, n, cs + cs_r0sq);
+      hipLaunchKernelGGL(coarse_cg_start_kernel, dim3(1), dim3(1), 0, stream, cs, done);
+      const unsigned int nb = grid_for(n, 256);
+      hipLaunchKernelGGL(diag_mult_kernel<double>, dim3(nb), dim3(256), 0, stream, z,
+                         l.inv_diag_d.get(), r, n);
+      copy(p, z, n);
+      dot_to(r, z, n, cs + cs_rz);
+      constexpr unsigned int check_every = 4;
+      for (unsigned int it = 0; it < lp.coarse_cg_max_iterations; ++it)
+        {
+          const int par = int(it & 1u);
+          A(ap, p);
+          dot_to(p, ap, n, cs + cs_pap);
+          hipLaunchKernelGGL(coarse_cg_xr_kernel, dim3(nb), dim3(256), 0, stream, x, r, p, ap,
+                             cs, done, par, n);
+          dot_to(r, r, n, cs + cs_rr);
+          hipLaunchKernelGGL(coarse_cg_check_kernel, dim3(1), dim3(1), 0, stream, cs, done,
+                             lp.coarse_cg_tolerance);
+          hipLaunchKernelGGL(diag_mult_kernel<double>, dim3(nb), dim3(256), 0, stream, z,
+                             l.inv_diag_d.get(), r, n);
+          dot_to(r, z, n, cs + cs_rz + (par ^ 1));
+          hipLaunchKernelGGL(coarse_cg_p_kernel, dim3(nb), dim3(256), 0, stream, p, z, cs, done,
+                             par, n);
+          if (it % check_every == check_every - 1)
+            {
+              int h = 0;
+              PPF_HIP_CHECK(hipMemcpyAsync(&h, done, sizeof(int), hipMemcpyDeviceToHost, stream));
+              sync();
+              if (h)
+                break;
+            }
+        }
+      PPF_HIP_CHECK(hipGetLastError());
       if constexpr (std::is_same_v<T, float>)
         convert(l.vf[0].get(), x, n);
     }
@@ -1957,6 +2039,14 @@ namespace ppf
     DeviceBuffer<float>                            zeros_f;
     DeviceBuffer<double>                           reduce_buf;
     DeviceBuffer<double>                           scalar_buf;
+    DeviceBuffer<double>                           eig_slots;
+    DeviceBuffer<double>                           coarse_slots;
+    /// Coarse-grid solve handed to the next lhs stencil launch (coarse_cg_kernel).
+    CoarseCgArgs                                   coarse_req {};
+    bool                                           coarse_pending = false;
+    /// Largest coarse lattice solved by one workgroup.
+    int64_t                                        coarse_fused_max_nodes = 4096;
+    DeviceBuffer<int>                              coarse_flag;
     PinnedBuffer<int>                              probe_flag; // two probe slots
     int                                            probe_tag[2] = {0, 0};
     unsigned int                                   probe_count  = 0;

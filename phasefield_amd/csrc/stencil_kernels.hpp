@@ -41,6 +41,7 @@
 #pragma once
 
 #include "operator_kernels.hpp"
+#include "vector_kernels.hpp"
 
 #include <type_traits>
 #include <utility>
@@ -676,6 +677,83 @@ namespace ppf
       run(std::integral_constant<bool, true> {});
     else
       run(std::integral_constant<bool, false> {});
+  }
+
+  /// Vectors and limits of a coarse-grid CG run by one workgroup.
+  struct CoarseCgArgs
+  {
+    double       *x, *r, *p, *ap, *z;
+    const double *inv_diag;
+    int64_t       n;       // block vector length
+    double        tol;     // stop at |r| <= tol |r0|
+    int           max_its;
+  };
+
+  /// The coarse-grid CG with the diagonal preconditioner (same recurrences and stopping
+  /// rule as the host-driven loop) in one workgroup: the level operator is evaluated
+  /// node by node (stencil_node, operand a.op = p, output out.dst = ap), dot products are
+  /// fixed-order block sums, and no host round trip or further launch is needed.
+  template <class PDE, int dim, int degree, int block_id, int NF, int NS>
+  __global__ void __launch_bounds__(reduce_threads)
+  coarse_cg_kernel(const PDE pde, const StencilArgs<double> a, const OutTable<double> out,
+                   const prismspf::SimulationTimer timer, const CoarseCgArgs c)
+  {
+    __shared__ double bcast;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    auto      dot = [&](const double *u, const double *v) {
+      double s = 0.0;
+      for (int64_t i = tid; i < c.n; i += nt)
+        s += u[i] * v[i];
+      s = block_sum(s);
+      if (tid == 0)
+        bcast = s;
+      __syncthreads();
+      const double r = bcast;
+      __syncthreads();
+      return r;
+    };
+    const int64_t n_nodes = int64_t(a.n[0]) * a.n[1] * a.n[2];
+    for (int64_t i = tid; i < c.n; i += nt)
+      c.x[i] = 0.0;
+    const double r0 = std::sqrt(dot(c.r, c.r));
+    if (!(r0 > 0.0))
+      return;
+    for (int64_t i = tid; i < c.n; i += nt)
+      {
+        c.z[i] = c.inv_diag[i] * c.r[i];
+        c.p[i] = c.z[i];
+      }
+    __syncthreads();
+    double rz = dot(c.r, c.z);
+    for (int it = 0; it < c.max_its; ++it)
+      {
+        for (int64_t node = tid; node < n_nodes; node += nt)
+          {
+            const int     x = int(node % a.n[0]);
+            const int64_t t = node / a.n[0];
+            stencil_node<PDE, dim, degree, double, true, block_id, NF, NS>(
+              pde, a, out, timer, x, int(t % a.n[1]), int(t / a.n[1]));
+          }
+        __syncthreads();
+        const double alpha = rz / dot(c.p, c.ap);
+        for (int64_t i = tid; i < c.n; i += nt)
+          {
+            c.x[i] = alpha * c.p[i] + 1.0 * c.x[i];
+            c.r[i] = (-alpha) * c.ap[i] + 1.0 * c.r[i];
+          }
+        __syncthreads();
+        if (std::sqrt(dot(c.r, c.r)) <= c.tol * r0)
+          break;
+        for (int64_t i = tid; i < c.n; i += nt)
+          c.z[i] = c.inv_diag[i] * c.r[i];
+        __syncthreads();
+        const double rz_new = dot(c.r, c.z);
+        const double beta   = rz_new / rz;
+        for (int64_t i = tid; i < c.n; i += nt)
+          c.p[i] = 1.0 * c.z[i] + beta * c.p[i];
+        rz = rz_new;
+        __syncthreads();
+      }
   }
 
   /// Writes the constraint values of the Dirichlet faces in `mask` into dst (explicit

@@ -146,6 +146,105 @@ namespace ppf
       out[0] = s;
   }
 
+  // ---------------------------------------------------- device-side CG scalars
+  //
+  // Small CG loops (eigenvalue estimates, coarse-grid solves) keep their scalars in
+  // device memory: dot products land in slots, alpha and beta are formed in the update
+  // kernels with the same expressions the host loop uses, and the host reads the slots
+  // once per estimate or every few coarse iterations instead of three times per step.
+
+  /// Iteration i of the eigenvalue estimate is valid if rz and pAp stayed positive up
+  /// to and including i (the host loop stops at the first that does not).
+  __device__ inline bool
+  eig_valid(const double *rz, const double *pap, int i)
+  {
+    for (int j = 0; j <= i; ++j)
+      if (!(rz[j] > 0.0) || !(pap[j] > 0.0))
+        return false;
+    return true;
+  }
+
+  /// r = (-alpha) ap + r with alpha = rz_i / pAp_i, if iteration i is valid.
+  template <typename T>
+  __global__ void
+  eig_r_update_kernel(T *__restrict__ r, const T *__restrict__ ap, const double *rz,
+                      const double *pap, int i, int64_t n)
+  {
+    if (!eig_valid(rz, pap, i))
+      return;
+    const double  alpha = rz[i] / pap[i];
+    const int64_t k     = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n)
+      r[k] = T(-alpha) * ap[k] + T(1) * r[k];
+  }
+
+  /// p = z + beta p with beta = rz_{i+1} / rz_i, if iteration i is valid.
+  template <typename T>
+  __global__ void
+  eig_p_update_kernel(T *__restrict__ p, const T *__restrict__ z, const double *rz,
+                      const double *pap, int i, int64_t n)
+  {
+    if (!eig_valid(rz, pap, i))
+      return;
+    const double  beta = rz[i + 1] / rz[i];
+    const int64_t k    = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n)
+      p[k] = T(1) * z[k] + T(beta) * p[k];
+  }
+
+  /// Slots of the coarse CG: |r0|^2, rz (two parities), pAp, |r|^2; flag[0] = done.
+  enum CoarseSlot : int
+  {
+    cs_r0sq = 0,
+    cs_rz   = 1,
+    cs_pap  = 3,
+    cs_rr   = 4
+  };
+
+  __global__ void
+  coarse_cg_start_kernel(const double *s, int *flag)
+  {
+    flag[0] = s[cs_r0sq] > 0.0 ? 0 : 1;
+  }
+
+  /// x = alpha p + x, r = (-alpha) ap + r with alpha = rz / pAp, unless done.
+  __global__ void
+  coarse_cg_xr_kernel(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+                      const double *__restrict__ ap, const double *s, const int *flag, int par,
+                      int64_t n)
+  {
+    if (flag[0])
+      return;
+    const double  alpha = s[cs_rz + par] / s[cs_pap];
+    const int64_t k     = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n)
+      {
+        x[k] = alpha * p[k] + 1.0 * x[k];
+        r[k] = (-alpha) * ap[k] + 1.0 * r[k];
+      }
+  }
+
+  /// Marks the solve done once |r| <= tol |r0|.
+  __global__ void
+  coarse_cg_check_kernel(const double *s, int *flag, double tol)
+  {
+    if (!flag[0] && std::sqrt(s[cs_rr]) <= tol * std::sqrt(s[cs_r0sq]))
+      flag[0] = 1;
+  }
+
+  /// p = z + beta p with beta = rz_new / rz, unless done.
+  __global__ void
+  coarse_cg_p_kernel(double *__restrict__ p, const double *__restrict__ z, const double *s,
+                     const int *flag, int par, int64_t n)
+  {
+    if (flag[0])
+      return;
+    const double  beta = s[cs_rz + (par ^ 1)] / s[cs_rz + par];
+    const int64_t k    = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n)
+      p[k] = 1.0 * z[k] + beta * p[k];
+  }
+
   /// CG update fused with the residual norm: x += alpha p, r += alpha Ap, partial |r|^2.
   template <typename T>
   __global__ void __launch_bounds__(reduce_threads)
