@@ -266,6 +266,22 @@ def main():
     del flush
     alg_bytes = 3 * 8 * n_d
     achieved = alg_bytes / t_kernel / 1e9
+    # Reference point at the same size: an elementwise add of two arrays into a third
+    # (3 arrays of the stored lattice, the c block's traffic without the stencil).
+    ea = torch.randn(info.n_nodes, dtype=torch.float64, device="cuda")
+    eb, ec = torch.randn_like(ea), torch.empty_like(ea)
+    for _ in range(5):
+        torch.add(ea, eb, out=ec)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.kernel_reps):
+        torch.add(ea, eb, out=ec)
+    e1.record()
+    e1.synchronize()
+    t_add = e0.elapsed_time(e1) / args.kernel_reps * 1e-3
+    add_gbs = 3 * 8 * info.n_nodes / t_add / 1e9
+    del ea, eb, ec
     roofline = {
         "bound": "hbm",
         "achieved": achieved,
@@ -279,6 +295,8 @@ def main():
         "achieved_flushed": alg_bytes / t_flushed / 1e9,
         "frac_flushed": alg_bytes / t_flushed / 1e9 / HBM_PEAK_GBS,
         "launch_seconds_flushed": t_flushed,
+        "elementwise_add_gbs": add_gbs,
+        "frac_of_elementwise_add": achieved / add_gbs,
     }
     del ctx, dst
     torch.cuda.empty_cache()

@@ -1025,6 +1025,8 @@ namespace ppf
       double                              lambda_max = 0.0, lambda_min = 0.0;
       std::vector<DeviceBuffer<double>>   field_d[PPF_MAX_FIELDS];
       std::vector<DeviceBuffer<float>>    field_f[PPF_MAX_FIELDS];
+      DeviceBuffer<double>                tmp_d[2]; // separable transfers to the coarser level
+      DeviceBuffer<float>                 tmp_f[2];
     };
 
     struct LinearState
@@ -1576,22 +1578,99 @@ namespace ppf
       for (std::size_t k = 0; k < blocks[b].field_indices.size(); ++k)
         {
           const int f = int(*std::next(blocks[b].field_indices.begin(), k));
-          hipLaunchKernelGGL((restrict_kernel<dim, degree, T>),
-                             dim3(grid_for(c.grid.n_nodes, 256)), dim3(256), 0, stream,
-                             lvec<T>(c, 1) + k * c.grid.n_nodes, c.grid,
-                             lvec<T>(l, 2) + k * l.grid.n_nodes, l.grid, dirichlet_mask(f));
+          restrict_separable<T>(lvec<T>(c, 1) + k * c.grid.n_nodes, c.grid,
+                                lvec<T>(l, 2) + k * l.grid.n_nodes, l, dirichlet_mask(f));
         }
       v_cycle<T>(b, li - 1);
       for (std::size_t k = 0; k < blocks[b].field_indices.size(); ++k)
         {
           const int f = int(*std::next(blocks[b].field_indices.begin(), k));
-          hipLaunchKernelGGL((prolongate_add_kernel<dim, degree, T>),
-                             dim3(grid_for(l.grid.n_nodes, 256)), dim3(256), 0, stream,
-                             lvec<T>(l, 0) + k * l.grid.n_nodes, l.grid,
-                             lvec<T>(c, 0) + k * c.grid.n_nodes, c.grid, dirichlet_mask(f));
+          prolongate_add_separable<T>(lvec<T>(l, 0) + k * l.grid.n_nodes, l,
+                                      lvec<T>(c, 0) + k * c.grid.n_nodes, c.grid,
+                                      dirichlet_mask(f));
         }
       chebyshev<T>(A, linv<T>(l), n, deg, l.lambda_max, l.lambda_min, lvec<T>(l, 0),
                    lvec<T>(l, 1), lvec<T>(l, 3), lvec<T>(l, 4), false);
+    }
+
+    /// Scratch lattices of the separable transfers between level l and the next
+    /// coarser one: [0] coarse x-extent by the fine y, z extents; [1] coarse x, y by fine z.
+    template <typename T>
+    T *
+    transfer_tmp(Level &l, int i, int64_t n)
+    {
+      auto &buf = [&]() -> auto & {
+        if constexpr (std::is_same_v<T, float>)
+          return l.tmp_f[i];
+        else
+          return l.tmp_d[i];
+      }();
+      if (buf.size() < n)
+        buf.resize(n);
+      return buf.get();
+    }
+
+    /// fine += P coarse (fine Dirichlet nodes untouched), one pass per axis.
+    template <typename T>
+    void
+    prolongate_add_separable(T *fine_v, Level &l, const T *coarse_v, const GridDesc &gc,
+                             unsigned int mask)
+    {
+      const GridDesc &gf = l.grid;
+      const int c0 = gc.nodes[0], c1 = gc.nodes[1], c2 = gc.nodes[2];
+      const int f0 = gf.nodes[0], f1 = gf.nodes[1], f2 = gf.nodes[2];
+      const T  *in = coarse_v;
+      if constexpr (dim == 3)
+        {
+          // z: (c0, c1, c2) -> (c0, c1, f2)
+          T *t1 = transfer_tmp<T>(l, 1, int64_t(c0) * c1 * f2);
+          hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, false, 0>),
+                             dim3(grid_for(int64_t(c0) * c1 * f2, 256)), dim3(256), 0, stream,
+                             t1, in, c0, c1, f2, c0, c1, 2, gc, gf, 0u);
+          in = t1;
+        }
+      // y: (c0, c1, f2) -> (c0, f1, f2)
+      T *t0 = transfer_tmp<T>(l, 0, int64_t(c0) * f1 * f2);
+      hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, false, 0>),
+                         dim3(grid_for(int64_t(c0) * f1 * f2, 256)), dim3(256), 0, stream, t0,
+                         in, c0, f1, f2, c0, c1, 1, gc, gf, 0u);
+      // x: (c0, f1, f2) -> fine, added
+      hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, false, 1>),
+                         dim3(grid_for(gf.n_nodes, 256)), dim3(256), 0, stream, fine_v, t0, f0,
+                         f1, f2, c0, f1, 0, gc, gf, mask);
+    }
+
+    /// coarse = P^T fine (coarse Dirichlet nodes zero), one pass per axis.
+    template <typename T>
+    void
+    restrict_separable(T *coarse_v, const GridDesc &gc, const T *fine_v, Level &l,
+                       unsigned int mask)
+    {
+      const GridDesc &gf = l.grid;
+      const int c0 = gc.nodes[0], c1 = gc.nodes[1], c2 = gc.nodes[2];
+      const int f0 = gf.nodes[0], f1 = gf.nodes[1], f2 = gf.nodes[2];
+      // x: (f0, f1, f2) -> (c0, f1, f2)
+      T *t0 = transfer_tmp<T>(l, 0, int64_t(c0) * f1 * f2);
+      hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, true, 0>),
+                         dim3(grid_for(int64_t(c0) * f1 * f2, 256)), dim3(256), 0, stream, t0,
+                         fine_v, c0, f1, f2, f0, f1, 0, gf, gc, 0u);
+      if constexpr (dim == 3)
+        {
+          // y: (c0, f1, f2) -> (c0, c1, f2)
+          T *t1 = transfer_tmp<T>(l, 1, int64_t(c0) * c1 * f2);
+          hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, true, 0>),
+                             dim3(grid_for(int64_t(c0) * c1 * f2, 256)), dim3(256), 0, stream,
+                             t1, t0, c0, c1, f2, c0, f1, 1, gf, gc, 0u);
+          // z: (c0, c1, f2) -> coarse, Dirichlet nodes zero
+          hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, true, 2>),
+                             dim3(grid_for(gc.n_nodes, 256)), dim3(256), 0, stream, coarse_v, t1,
+                             c0, c1, c2, c0, c1, 2, gf, gc, mask);
+        }
+      else
+        // y: (c0, f1) -> coarse, Dirichlet nodes zero
+        hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, true, 2>),
+                           dim3(grid_for(gc.n_nodes, 256)), dim3(256), 0, stream, coarse_v, t0,
+                           c0, c1, c2, c0, f1, 1, gf, gc, mask);
     }
 
     template <typename T>
