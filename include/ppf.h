@@ -105,7 +105,9 @@ int ppf_field_name(ppf_ctx *ctx, int field, char *buf, int len);
  * "mg_level_precision" = double|single, "mg_coarse_solver" = smoother|cg,
  * "linear_tolerance" = <value> and "linear_max_iterations" = <n> (applied to every
  * linear block, overriding the parameter file); "linear_initial_guess" =
- * previous|zero (previous: the last solution, as the reference). */
+ * previous|zero (previous: the last solution, as the reference); "explicit_precision" =
+ * double|single (single: explicit blocks compute in FP32 from FP32 copies of their
+ * inputs and store into the FP64 state). */
 int ppf_set_option(ppf_ctx *ctx, const char *key, const char *value);
 
 /* Field state `age` (0 = current, k = old_k, PPF_AGE_LATEST = the newest values) from/to

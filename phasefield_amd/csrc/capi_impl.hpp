@@ -255,6 +255,12 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
         for (auto &b : c.rt.blocks)
           b.linear_solver_parameters.max_iterations = unsigned(std::stoul(v));
       }
+    else if (k == "explicit_precision")
+      {
+        if (v != "double" && v != "single")
+          throw ppf::Error(ppf::status_invalid, "explicit_precision must be double|single");
+        c.rt.explicit_fp32 = v == "single";
+      }
     else if (k == "linear_initial_guess")
       {
         if (v != "previous" && v != "zero")

@@ -1006,11 +1006,65 @@ namespace ppf
     void
     solve_explicit(int b)
     {
+      if (explicit_fp32)
+        {
+          solve_explicit_fp32(b);
+          return;
+        }
       std::vector<double *> dst;
       for (auto f : blocks[b].field_indices)
         dst.push_back(field_ptr(int(f), 0));
       apply_operator<double>(b, false, fine_fields(), {}, dst, scale_invm, dirichlet_value,
                              false, &dirichlet_values);
+    }
+
+    /// Explicit update with FP32 operator arithmetic and FP64 state (option
+    /// explicit_precision = single): the rhs inputs are rounded to FP32 copies, the FP32
+    /// instantiation of the equations computes the new values, and those are stored
+    /// back into the FP64 field state.
+    void
+    solve_explicit_fp32(int b)
+    {
+      const auto        &blk = blocks[b];
+      const int64_t      n   = fine.n_nodes;
+      LevelFields<float> lf {};
+      lf.grid                = &fine;
+      constexpr int src_slot = prismspf::dependency_slot(prismspf::SRC);
+      explicit_in_f.resize(fields.size() * (PPF_MAX_AGE + 1));
+      for (const auto &in : declared_inputs(blk.dependencies_rhs))
+        {
+          if (in.slot == src_slot || lf.ptr[in.field][in.slot])
+            continue;
+          const int age = in.slot - 1;
+          if (age < 0 || age >= int(storage[in.field].size()))
+            throw Error(status_internal, "dependency storage not available");
+          auto &buf = explicit_in_f[in.field * (PPF_MAX_AGE + 1) + age];
+          buf.resize(n);
+          convert(buf.get(), storage[in.field][age].get(), n);
+          lf.ptr[in.field][in.slot] = buf.get();
+        }
+      if (dirichlet_values_f.size() != dirichlet_values.size())
+        {
+          dirichlet_values_f.clear();
+          dirichlet_values_f.resize(dirichlet_values.size());
+          for (std::size_t f = 0; f < dirichlet_values.size(); ++f)
+            if (dirichlet_values[f].get())
+              {
+                dirichlet_values_f[f].resize(n);
+                convert(dirichlet_values_f[f].get(), dirichlet_values[f].get(), n);
+              }
+        }
+      explicit_out_f.resize(fields.size());
+      std::vector<float *> dst;
+      for (auto f : blk.field_indices)
+        {
+          explicit_out_f[f].resize(n);
+          dst.push_back(explicit_out_f[f].get());
+        }
+      apply_operator<float>(b, false, lf, {}, dst, scale_invm, dirichlet_value, false,
+                            &dirichlet_values_f);
+      for (auto f : blk.field_indices)
+        convert(field_ptr(int(f), 0), explicit_out_f[f].get(), n);
     }
 
     // ------------------------------------------------------------- linear solve
@@ -1953,7 +2007,8 @@ namespace ppf
     bool
     speculable(unsigned int inc) const
     {
-      if (inc == 0 || inputs.output_parameters.should_output(inc) || pde_d->get_user_stop())
+      if (inc == 0 || inputs.output_parameters.should_output(inc) || pde_d->get_user_stop() ||
+          explicit_fp32)
         return false;
       for (const auto &blk : blocks)
         {
@@ -2107,6 +2162,9 @@ namespace ppf
     /// Start every linear solve from zero instead of the previous solution (which the
     /// reference keeps as the next initial guess, linear_solver.h:330-336).
     bool                                   zero_initial_guess    = false;
+    /// FP32 operator arithmetic for explicit blocks (FP64 state), with its buffers.
+    bool                                   explicit_fp32         = false;
+    std::vector<DeviceBuffer<float>>       explicit_in_f, explicit_out_f, dirichlet_values_f;
 
     std::vector<std::vector<DeviceBuffer<double>>> storage;
     std::vector<DeviceBuffer<double>>              dirichlet_values;

@@ -201,6 +201,34 @@ def test_run_matches_oracle(app, text, steps):
         assert abs(a[1] - b[1]) <= 1e-10 * max(1.0, abs(b[1]))
 
 
+FP32_EXPLICIT_CASES = [
+    ("precipitate_3d_q1", prm.precipitate(size=16.0, refinement=3, subdivisions=(1, 1, 1), steps=20, outputs=1), 20),
+    ("cahn_hilliard_3d_q2", prm.cahn_hilliard(size=8.0, refinement=2, steps=20, dt=1e-3, outputs=1), 20),
+]
+
+
+@pytest.mark.parametrize("app,text,steps", FP32_EXPLICIT_CASES, ids=[c[0] for c in FP32_EXPLICIT_CASES])
+def test_fp32_operator_fp64_state_error(app, text, steps):
+    """explicit_precision = single: FP32 operator arithmetic, FP64 state.  Error against
+    the FP64 oracle after the run (max over fields of max |error| / max |field|): at most
+    1e-4 (FP32 rounding of the inputs of every increment; the CH mu block's Laplacian of
+    c ~ 0.5 cancels to a few 1e-5, the precipitate model stays near 1e-6), and above the
+    FP64 path's error (the FP32 path really ran)."""
+    orc = Oracle(app, text)
+    assert orc.run(steps) == 0
+    errs = {}
+    for precision in ("double", "single"):
+        ctx = Context(app, text)
+        ctx.set_option("explicit_precision", precision)
+        assert ctx.run(steps) == 0
+        errs[precision] = max(rel_err(ctx.get_field(f, AGE_LATEST).ravel(), orc.get_field(f, AGE_LATEST))
+                              for f in range(ctx.info.n_fields))
+    print(app, "error vs FP64 oracle:", errs, flush=True)
+    assert errs["double"] <= 1e-10
+    assert errs["single"] <= 1e-4
+    assert errs["single"] > errs["double"]
+
+
 LINEAR_CASES = [
     ("poisson_2d_q2", dict(dim=2, refinement=4, preconditioner="None", mg_depth=1)),
     ("poisson_2d_q2", dict(dim=2, refinement=4, preconditioner="Chebyshev", mg_depth=1)),
