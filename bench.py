@@ -97,14 +97,15 @@ def time_kernel(ctx, torch, block_id, which, scaling, d_src, d_dst, reps, flush=
     return total / reps * 1e-3  # seconds
 
 
-def profiled_traffic(name):
-    """HBM bytes per launch from a committed rocprofv3 counter summary (FETCH_SIZE and
-    WRITE_SIZE scaled by the copy-kernel calibration); None if the summary is absent."""
+def profiled(name, key="traffic_bytes_per_launch"):
+    """A per-launch figure from a committed rocprofv3 counter summary: HBM bytes
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration) or FP64 flops (VALU
+    FMA/ADD/MUL instruction counts); None if the summary or the figure is absent."""
     path = Path(__file__).resolve().parent / "profiles" / name
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -165,8 +166,11 @@ def poisson_extra(torch):
         "bytes_per_launch": bytes_apply,
         "achieved_gbs": bytes_apply / t_apply / 1e9,
         "frac": bytes_apply / t_apply / 1e9 / HBM_PEAK_GBS,
-        "traffic": profiled_traffic("stencil_poisson_q2_128_fp64_pmc.md"),
+        "traffic": profiled("stencil_poisson_q2_128_fp64_pmc.md"),
     }
+    pflops = profiled("stencil_poisson_q2_128_fp64_pmc.md", "fp64_flops_per_launch")
+    if pflops is not None:
+        out["fp64_apply"]["fp64_tflops"] = pflops / t_apply / 1e12
     del x, y
     # The same application in FP32 (the multigrid levels' arithmetic).
     from phasefield_amd.capi import FP32
@@ -288,7 +292,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": profiled_traffic("stencil_cahn_hilliard_q2_64_fp64_pmc.md"),
+        "traffic": profiled("stencil_cahn_hilliard_q2_64_fp64_pmc.md"),
         "kernel": "stencil_operator_kernel (Cahn-Hilliard c block, Q2, FP64)",
         "launch_seconds": t_kernel,
         "bytes_per_launch": alg_bytes,
@@ -298,6 +302,11 @@ def main():
         "elementwise_add_gbs": add_gbs,
         "frac_of_elementwise_add": achieved / add_gbs,
     }
+    flops = profiled("stencil_cahn_hilliard_q2_64_fp64_pmc.md", "fp64_flops_per_launch")
+    if flops is not None:
+        roofline["fp64_flops_per_launch"] = flops
+        roofline["fp64_tflops"] = flops / t_kernel / 1e12
+        roofline["fp64_frac_of_peak"] = flops / t_kernel / 1e12 / FP64_PEAK_TFLOPS
     del ctx, dst
     torch.cuda.empty_cache()
 
