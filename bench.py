@@ -16,9 +16,11 @@ Printed JSON line:
   cpu_baseline the CPU oracle (FP64 restatement of the reference algorithm, OpenMP) on
                the host cores, timed on a bounded number of increments of the same
                configuration;
-  extra        Poisson 3D Q2 128^3: FP64 operator application roofline and
-               conjugate-gradient + geometric-multigrid time to 1e-8 relative residual,
-               all-FP64 versus FP32 levels with FP64 outer iteration.
+  extra        Poisson 3D Q2 128^3: FP64 and FP32 operator application rooflines and
+               conjugate-gradient + geometric-multigrid time to 1e-8 and 1e-10 relative
+               residual, all-FP64 versus FP32 levels with FP64 outer iteration; Allen-Cahn
+               3D adjoint: forward and vector-Jacobian-product times (FP64 and
+               FP32-forward) with their algorithmic GB/s (16 and 40 B per DoF and step).
 """
 
 from __future__ import annotations
@@ -226,6 +228,48 @@ def poisson_extra(torch):
     return out
 
 
+def adjoint_extra(torch):
+    """Allen-Cahn 3D adjoint configuration: Q1, 128^3 cells, 500 steps, checkpoints every
+    50 steps; forward run and vector-Jacobian product (recompute + adjoint sweep)."""
+    from phasefield_amd import Context, prm
+    from phasefield_amd.capi import AGE_LATEST, FP32, FP64
+
+    ctx = Context("allen_cahn_3d_q1", prm.config_prm("allen_cahn_adjoint_3d"))
+    ctx.run(0)  # seeded initial condition
+    info = ctx.info
+    n_d, n, every = info.n_dofs, 500, 50
+    full = ctx.get_field(0, AGE_LATEST).ravel()
+    dims = tuple(reversed([info.degree * info.cells[d] + 1 for d in range(info.dim)]))
+    stored = np.ascontiguousarray(full.reshape(dims)[: info.nodes[2], : info.nodes[1], : info.nodes[0]]).ravel()
+    phi0 = torch.tensor(stored, dtype=torch.float64, device="cuda")
+    out = torch.empty_like(phi0)
+    lam = torch.randn_like(phi0)
+    dphi = torch.empty_like(phi0)
+    res = {"workload": "Allen-Cahn 3D adjoint, Q1, 128^3 cells, 500 steps of dt=0.01, "
+                       "checkpoints every 50 steps"}
+    for label, prec in (("fp64", FP64), ("fp32_forward", FP32)):
+        ctx.adjoint_forward(phi0.data_ptr(), 1.0, 1.5, 5, prec, out.data_ptr())  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.adjoint_forward(phi0.data_ptr(), 1.0, 1.5, n, prec, out.data_ptr())
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ctx.adjoint_vjp(phi0.data_ptr(), lam.data_ptr(), 1.0, 1.5, n, every, prec, dphi.data_ptr())
+        torch.cuda.synchronize()
+        t_vjp = time.perf_counter() - t0
+        res[label] = {
+            "forward_seconds": t_fwd,
+            "forward_us_per_step": t_fwd / n * 1e6,
+            "forward_gbs": 16 * n_d * n / t_fwd / 1e9,
+            "vjp_seconds": t_vjp,
+            "vjp_us_per_step": t_vjp / n * 1e6,
+            "vjp_gbs": 40 * n_d * n / t_vjp / 1e9,
+        }
+    del ctx
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -317,7 +361,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    extra = None if args.no_extra else poisson_extra(torch)
+    extra = None
+    if not args.no_extra:
+        extra = poisson_extra(torch)
+        torch.cuda.empty_cache()
+        extra["adjoint"] = adjoint_extra(torch)
     cpu = cpu_baseline(args.cpu_seconds) if args.cpu_seconds > 0 else None
 
     line = {
