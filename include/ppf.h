@@ -183,7 +183,10 @@ int ppf_synchronize(ppf_ctx *ctx);
  *
  * ppf_adjoint_vjp: given lam_final = dJ/dphi_N for any loss J, returns dJ/dL,
  * dJ/dkappa and dJ/dphi0, running the adjoint steps backwards in time and recomputing
- * forward states from checkpoints stored every `checkpoint_every` steps.
+ * forward states from checkpoints stored every `checkpoint_every` steps.  The
+ * checkpoints (ceil(n_steps / checkpoint_every) states) and the recomputed segment
+ * (checkpoint_every states) stay allocated in the context for later calls, until
+ * ppf_destroy.
  *
  * ppf_lumped_mass: the lumped mass m_i of every stored node (for losses such as
  * J = 1/2 sum_i m_i (phi_N,i - target_i)^2). */

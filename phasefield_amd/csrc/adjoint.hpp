@@ -18,73 +18,248 @@
 
 #include "device.hpp"
 #include "grid.hpp"
-#include "operator_kernels.hpp"
-#include "vector_kernels.hpp"
+#include "stencil_kernels.hpp"
 
 namespace ppf
 {
-  /// One forward step in arithmetic T; state stored in FP64.
-  template <int dim, int degree, typename T, class Energy>
-  __global__ void __launch_bounds__(block_size)
-  ac_forward_kernel(double *__restrict__ out, const double *__restrict__ phi, const GridDesc g,
-                    double dt, double L, double kappa)
+  /// Lattice of the column kernels below.  With GLL collocation the assembled
+  /// stiffness row of a node is its lumped mass times L u = sum_d sum_o W_d(i_d, o)
+  /// u_{i + o e_d} (the interior taps W of the stencil kernel, mirror ghosts at
+  /// non-periodic faces), so M^{-1} K phi = L phi and K (M^{-1} lam) = M L (lam / M).
+  struct AdjointLattice
   {
-    const int64_t node = int64_t(blockIdx.x) * block_size + threadIdx.x;
-    if (node >= g.n_nodes)
-      return;
-    int i[3];
-    decompose<dim>(g, node, g.nodes, i);
-    const T      p   = T(phi[node]);
-    const T      kp  = T(stiffness_row<dim, degree, double>(g, i, phi, false, 0u));
-    const double m   = lumped_mass<dim, degree>(g, i);
-    const T      upd = p - T(dt * L) * Energy::template fprime<T>(p) - T(dt * L * kappa / m) * kp;
-    out[node]        = double(upd);
+    int    n[3];
+    int    periodic[3];
+    int    plane;   ///< n[0] * n[1]
+    int    z_chunk; ///< planes per column segment (one workgroup row of the grid)
+    double tap[3][2][3];
+    double mass[3][3];     ///< 1-D lumped mass per scale class
+    double inv_mass[3][3]; ///< its inverse
+  };
+
+  template <int dim, int degree>
+  inline AdjointLattice
+  make_adjoint_lattice(const GridDesc &g)
+  {
+    AdjointLattice a {};
+    const double tq2[2][3] = {{14.0, -8.0, 1.0}, {8.0, -4.0, 0.0}};
+    const double tq1[3]    = {2.0, -1.0, 0.0};
+    for (int d = 0; d < 3; ++d)
+      {
+        a.n[d]        = d < dim ? g.nodes[d] : 1;
+        a.periodic[d] = d < dim ? g.periodic[d] : 0;
+        for (int par = 0; par < 2; ++par)
+          for (int k = 0; k < 3; ++k)
+            a.tap[d][par][k] =
+              d < dim ? (degree == 2 ? tq2[par][k] : tq1[k]) / (g.h[d] * g.h[d]) : 0.0;
+        for (int c = 0; c < 3; ++c)
+          {
+            a.mass[d][c]     = d < dim ? class_mass_units<degree>(c) * g.h[d] : 1.0;
+            a.inv_mass[d][c] = 1.0 / a.mass[d][c];
+          }
+      }
+    a.plane = a.n[0] * a.n[1];
+    // Enough column segments for ~2048 workgroups (the GPU holds 256 CUs).
+    const int blocks_x = (a.plane + block_size - 1) / block_size;
+    const int chunks   = std::max(1, std::min(a.n[2], (2048 + blocks_x - 1) / blocks_x));
+    a.z_chunk          = (a.n[2] + chunks - 1) / chunks;
+    return a;
   }
 
-  /// One adjoint step: lam_prev = dF/dphi(phi)^T lam, plus per-block partial sums of
-  /// lam . dF/dL and lam . dF/dkappa.  `lam_over_m` holds lam / m.
+  /// Per-column constants: the in-plane offsets of the x and y neighbours (source nodes
+  /// after wrap / mirror), their taps by the column's parity, and 1-D inverse masses.
+  template <int degree, typename T>
+  struct AdjointColumn
+  {
+    static constexpr int RAD = degree;
+    int                  xoff[2 * RAD + 1], yoff[2 * RAD + 1];
+    T                    tx[RAD + 1], ty[RAD + 1];
+    double               ixm[2 * RAD + 1], iym[2 * RAD + 1];
+    double               mx, my;
+
+    __device__ PPF_INLINE
+    AdjointColumn(const AdjointLattice &a, int x, int y)
+    {
+#pragma unroll
+      for (int k = 0; k <= 2 * RAD; ++k)
+        {
+          const int xs = source_coord(x + k - RAD, a.n[0], a.periodic[0]);
+          const int ys = source_coord(y + k - RAD, a.n[1], a.periodic[1]);
+          xoff[k]      = y * a.n[0] + xs;
+          yoff[k]      = ys * a.n[0] + x;
+          ixm[k]       = a.inv_mass[0][scale_class<degree>(xs, a.n[0], a.periodic[0])];
+          iym[k]       = a.inv_mass[1][scale_class<degree>(ys, a.n[1], a.periodic[1])];
+        }
+#pragma unroll
+      for (int o = 0; o <= RAD; ++o)
+        {
+          tx[o] = T(a.tap[0][degree == 2 && (x & 1)][o]);
+          ty[o] = T(a.tap[1][degree == 2 && (y & 1)][o]);
+        }
+      mx = a.mass[0][scale_class<degree>(x, a.n[0], a.periodic[0])];
+      my = a.mass[1][scale_class<degree>(y, a.n[1], a.periodic[1])];
+    }
+  };
+
+  /// One forward step in arithmetic T (state stored in FP64):
+  /// phi' = phi - dt L f'(phi) - dt L kappa L phi.  A thread owns one (x, y) column
+  /// of a z segment; z neighbours come from a register ring, x and y neighbours from
+  /// the plane (cached: the neighbouring lanes load them as their centres).
+  template <int dim, int degree, typename T, class Energy>
+  __global__ void __launch_bounds__(block_size)
+  ac_forward_kernel(double *__restrict__ out, const double *__restrict__ phi,
+                    const AdjointLattice a, double dt, double L, double kappa)
+  {
+    constexpr int RAD = degree;
+    const int     q   = int(blockIdx.x) * block_size + int(threadIdx.x);
+    if (q >= a.plane)
+      return;
+    const int                   x = q % a.n[0], y = q / a.n[0];
+    const AdjointColumn<degree, T> col(a, x, y);
+    const int z0 = dim == 3 ? int(blockIdx.y) * a.z_chunk : 0;
+    const int z1 = dim == 3 ? min(z0 + a.z_chunk, a.n[2]) : 1;
+    const T   c1 = T(dt * L), c2 = T(dt * L * kappa);
+
+    T ring[2 * RAD + 1];
+    if constexpr (dim == 3)
+#pragma unroll
+      for (int k = 0; k < 2 * RAD; ++k)
+        ring[k + 1] = T(phi[int64_t(source_coord(z0 + k - RAD, a.n[2], a.periodic[2])) * a.plane + q]);
+    for (int z = z0; z < z1; ++z)
+      {
+        const double *pl = phi + int64_t(z) * a.plane;
+        T             u;
+        T             lap;
+        if constexpr (dim == 3)
+          {
+#pragma unroll
+            for (int k = 0; k < 2 * RAD; ++k)
+              ring[k] = ring[k + 1];
+            ring[2 * RAD] =
+              T(phi[int64_t(source_coord(z + RAD, a.n[2], a.periodic[2])) * a.plane + q]);
+            u                = ring[RAD];
+            const double *tz = a.tap[2][degree == 2 && (z & 1)];
+            lap              = (col.tx[0] + col.ty[0] + T(tz[0])) * u;
+#pragma unroll
+            for (int o = 1; o <= RAD; ++o)
+              lap += T(tz[o]) * (ring[RAD - o] + ring[RAD + o]);
+          }
+        else
+          {
+            u   = T(pl[q]);
+            lap = (col.tx[0] + col.ty[0]) * u;
+          }
+#pragma unroll
+        for (int o = 1; o <= RAD; ++o)
+          {
+            lap += col.tx[o] * (T(pl[col.xoff[RAD - o]]) + T(pl[col.xoff[RAD + o]]));
+            lap += col.ty[o] * (T(pl[col.yoff[RAD - o]]) + T(pl[col.yoff[RAD + o]]));
+          }
+        out[int64_t(z) * a.plane + q] = double(u - c1 * Energy::template fprime<T>(u) - c2 * lap);
+      }
+  }
+
+  /// One adjoint step, lam_prev = dF/dphi(phi)^T lam
+  ///   = lam - dt L f''(phi) lam - dt L kappa M L(lam / M),
+  /// with lam . dF/dL and lam . dF/dkappa (dF/dL = -dt f'(phi) - dt kappa L phi,
+  /// dF/dkappa = -dt L L phi) added to per-workgroup sums (partial[b], partial[count + b]),
+  /// which the host reduces once after the sweep.  Same column traversal as the forward
+  /// step; lam / M is formed as lam is loaded.
   template <int dim, int degree, class Energy>
   __global__ void __launch_bounds__(block_size)
   ac_adjoint_kernel(double *__restrict__ lam_prev, double *__restrict__ partial,
-                    const double *__restrict__ lam, const double *__restrict__ lam_over_m,
-                    const double *__restrict__ phi, const GridDesc g, double dt, double L,
-                    double kappa)
+                    const double *__restrict__ lam, const double *__restrict__ phi,
+                    const AdjointLattice a, double dt, double L, double kappa)
   {
-    const int64_t node = int64_t(blockIdx.x) * block_size + threadIdx.x;
+    constexpr int RAD = degree;
+    const int     q   = int(blockIdx.x) * block_size + int(threadIdx.x);
     double        gL = 0.0, gk = 0.0;
-    if (node < g.n_nodes)
+    if (q < a.plane)
       {
-        int i[3];
-        decompose<dim>(g, node, g.nodes, i);
-        const double p   = phi[node];
-        const double l   = lam[node];
-        const double m   = lumped_mass<dim, degree>(g, i);
-        const double kp  = stiffness_row<dim, degree, double>(g, i, phi, false, 0u);
-        const double klm = stiffness_row<dim, degree, double>(g, i, lam_over_m, false, 0u);
-        lam_prev[node]   = l - dt * L * Energy::template fprime2<double>(p) * l - dt * L * kappa * klm;
-        gL               = l * (-dt * Energy::template fprime<double>(p) - dt * kappa * kp / m);
-        gk               = l * (-dt * L * kp / m);
+        const int x = q % a.n[0], y = q / a.n[0];
+        const AdjointColumn<degree, double> col(a, x, y);
+        const int z0 = dim == 3 ? int(blockIdx.y) * a.z_chunk : 0;
+        const int z1 = dim == 3 ? min(z0 + a.z_chunk, a.n[2]) : 1;
+        const double ixy = col.ixm[RAD] * col.iym[RAD];
+
+        // Rings of phi and lam / M along z (lam / M uses the mass of the source node).
+        double rp[2 * RAD + 1], rl[2 * RAD + 1];
+        auto   load = [&](int k, int zz) {
+          const int zs = source_coord(zz, a.n[2], a.periodic[2]);
+          const int64_t i = int64_t(zs) * a.plane + q;
+          rp[k] = phi[i];
+          rl[k] = lam[i] * (ixy * a.inv_mass[2][scale_class<degree>(zs, a.n[2], a.periodic[2])]);
+        };
+        if constexpr (dim == 3)
+#pragma unroll
+          for (int k = 0; k < 2 * RAD; ++k)
+            load(k + 1, z0 + k - RAD);
+        for (int z = z0; z < z1; ++z)
+          {
+            const int64_t base = int64_t(z) * a.plane;
+            const double *pp = phi + base, *pl = lam + base;
+            double        p, lom, lp, ll, mz;
+            if constexpr (dim == 3)
+              {
+#pragma unroll
+                for (int k = 0; k < 2 * RAD; ++k)
+                  {
+                    rp[k] = rp[k + 1];
+                    rl[k] = rl[k + 1];
+                  }
+                load(2 * RAD, z + RAD);
+                p                = rp[RAD];
+                lom              = rl[RAD];
+                const double *tz = a.tap[2][degree == 2 && (z & 1)];
+                lp               = (col.tx[0] + col.ty[0] + tz[0]) * p;
+                ll               = (col.tx[0] + col.ty[0] + tz[0]) * lom;
+#pragma unroll
+                for (int o = 1; o <= RAD; ++o)
+                  {
+                    lp += tz[o] * (rp[RAD - o] + rp[RAD + o]);
+                    ll += tz[o] * (rl[RAD - o] + rl[RAD + o]);
+                  }
+                mz = a.mass[2][scale_class<degree>(z, a.n[2], a.periodic[2])];
+              }
+            else
+              {
+                p   = pp[q];
+                lom = pl[q] * ixy;
+                lp  = (col.tx[0] + col.ty[0]) * p;
+                ll  = (col.tx[0] + col.ty[0]) * lom;
+                mz  = 1.0;
+              }
+            const double imz = 1.0 / mz;
+            double       sx = 0.0, sy = 0.0;
+#pragma unroll
+            for (int o = 1; o <= RAD; ++o)
+              {
+                lp += col.tx[o] * (pp[col.xoff[RAD - o]] + pp[col.xoff[RAD + o]]);
+                lp += col.ty[o] * (pp[col.yoff[RAD - o]] + pp[col.yoff[RAD + o]]);
+                sx += col.tx[o] * (pl[col.xoff[RAD - o]] * col.ixm[RAD - o] +
+                                   pl[col.xoff[RAD + o]] * col.ixm[RAD + o]);
+                sy += col.ty[o] * (pl[col.yoff[RAD - o]] * col.iym[RAD - o] +
+                                   pl[col.yoff[RAD + o]] * col.iym[RAD + o]);
+              }
+            ll += imz * (col.iym[RAD] * sx + col.ixm[RAD] * sy);
+            const double m = col.mx * col.my * mz;
+            const double l = pl[q];
+            lam_prev[base + q] =
+              l - dt * L * Energy::template fprime2<double>(p) * l - dt * L * kappa * m * ll;
+            gL += l * (-dt * Energy::template fprime<double>(p) - dt * kappa * lp);
+            gk += l * (-dt * L * lp);
+          }
       }
     gL = block_sum(gL);
     __syncthreads();
     gk = block_sum(gk);
     if (threadIdx.x == 0)
       {
-        partial[blockIdx.x]             = gL;
-        partial[gridDim.x + blockIdx.x] = gk;
+        const int count = int(gridDim.x * gridDim.y);
+        const int b     = int(blockIdx.x + gridDim.x * blockIdx.y);
+        partial[b] += gL;
+        partial[count + b] += gk;
       }
-  }
-
-  template <int dim, int degree>
-  __global__ void
-  divide_by_mass_kernel(double *__restrict__ out, const double *__restrict__ in, const GridDesc g)
-  {
-    const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (node >= g.n_nodes)
-      return;
-    int i[3];
-    decompose<dim>(g, node, g.nodes, i);
-    out[node] = in[node] / lumped_mass<dim, degree>(g, i);
   }
 
   template <int dim, int degree>
@@ -114,8 +289,8 @@ namespace ppf
     b = block_sum(b);
     if (threadIdx.x == 0)
       {
-        out[0] += a;
-        out[1] += b;
+        out[0] = a;
+        out[1] = b;
       }
   }
 
@@ -126,19 +301,25 @@ namespace ppf
   public:
     AllenCahnAdjoint(const GridDesc &g_, hipStream_t s_)
       : g(g_)
+      , lat(make_adjoint_lattice<dim, degree>(g_))
       , stream(s_)
-    {}
+    {
+      // Column kernels: int in-plane offsets and a z-segment grid row per chunk.
+      if (int64_t(g.nodes[0]) * (dim > 1 ? g.nodes[1] : 1) >= (int64_t(1) << 31))
+        throw Error(status_unsupported, "adjoint: lattice plane exceeds 2^31 nodes");
+      grid = dim3((lat.plane + block_size - 1) / block_size,
+                  dim == 3 ? (lat.n[2] + lat.z_chunk - 1) / lat.z_chunk : 1);
+    }
 
     void
     step(double *out, const double *phi, double dt, double L, double kappa, int precision)
     {
-      const unsigned int nb = grid_for(g.n_nodes, block_size);
       if (precision == 1)
-        hipLaunchKernelGGL((ac_forward_kernel<dim, degree, float, Energy>), dim3(nb),
-                           dim3(block_size), 0, stream, out, phi, g, dt, L, kappa);
+        hipLaunchKernelGGL((ac_forward_kernel<dim, degree, float, Energy>), grid,
+                           dim3(block_size), 0, stream, out, phi, lat, dt, L, kappa);
       else
-        hipLaunchKernelGGL((ac_forward_kernel<dim, degree, double, Energy>), dim3(nb),
-                           dim3(block_size), 0, stream, out, phi, g, dt, L, kappa);
+        hipLaunchKernelGGL((ac_forward_kernel<dim, degree, double, Energy>), grid,
+                           dim3(block_size), 0, stream, out, phi, lat, dt, L, kappa);
     }
 
     /// phi_final = F^n(phi0).
@@ -184,10 +365,11 @@ namespace ppf
       segment.resize(every);
       for (auto &s : segment)
         s.resize(g.n_nodes);
-      ensure(3);
-      partial.resize(2 * reduce_blocks_for());
+      ensure(2);
+      const int n_blocks = int(grid.x * grid.y);
+      partial.resize(2 * n_blocks);
       sums.resize(2);
-      PPF_HIP_CHECK(hipMemsetAsync(sums.get(), 0, 2 * sizeof(double), stream));
+      PPF_HIP_CHECK(hipMemsetAsync(partial.get(), 0, 2 * n_blocks * sizeof(double), stream));
 
       // Forward sweep storing the checkpoints phi_{k*every}, k = 0 .. n_ckpt-1.
       PPF_HIP_CHECK(hipMemcpyAsync(checkpoints[0].get(), phi0, sizeof(double) * g.n_nodes,
@@ -205,10 +387,8 @@ namespace ppf
       // Reverse sweep.
       double *lam  = work[0].get();
       double *lamp = work[1].get();
-      double *lom  = work[2].get();
       PPF_HIP_CHECK(hipMemcpyAsync(lam, lam_final, sizeof(double) * g.n_nodes,
                                    hipMemcpyDeviceToDevice, stream));
-      const unsigned int nb = grid_for(g.n_nodes, block_size);
       for (int seg = int(n_ckpt) - 1; seg >= 0; --seg)
         {
           const unsigned int first = unsigned(seg) * every;
@@ -222,16 +402,15 @@ namespace ppf
                  precision);
           for (int s = int(last) - 1; s >= int(first); --s)
             {
-              hipLaunchKernelGGL((divide_by_mass_kernel<dim, degree>), dim3(nb),
-                                 dim3(block_size), 0, stream, lom, lam, g);
-              hipLaunchKernelGGL((ac_adjoint_kernel<dim, degree, Energy>), dim3(nb),
-                                 dim3(block_size), 0, stream, lamp, partial.get(), lam, lom,
-                                 segment[s - first].get(), g, dt, L, kappa);
-              hipLaunchKernelGGL(reduce_two_kernel, dim3(1), dim3(reduce_threads), 0, stream,
-                                 sums.get(), partial.get(), int(nb));
+              hipLaunchKernelGGL((ac_adjoint_kernel<dim, degree, Energy>), grid,
+                                 dim3(block_size), 0, stream, lamp, partial.get(), lam,
+                                 segment[s - first].get(), lat, dt, L, kappa);
               std::swap(lam, lamp);
             }
         }
+      // Per-workgroup sums over all steps, then workgroups in index order.
+      hipLaunchKernelGGL(reduce_two_kernel, dim3(1), dim3(reduce_threads), 0, stream, sums.get(),
+                         partial.get(), n_blocks);
       PPF_HIP_CHECK(hipMemcpyAsync(dphi0, lam, sizeof(double) * g.n_nodes,
                                    hipMemcpyDeviceToDevice, stream));
       double host[2];
@@ -243,12 +422,6 @@ namespace ppf
     }
 
   private:
-    int
-    reduce_blocks_for() const
-    {
-      return int(grid_for(g.n_nodes, block_size));
-    }
-
     void
     ensure(int k)
     {
@@ -260,6 +433,8 @@ namespace ppf
     }
 
     GridDesc                          g;
+    AdjointLattice                    lat;
+    dim3                              grid;
     hipStream_t                       stream;
     std::vector<DeviceBuffer<double>> work, checkpoints, segment;
     DeviceBuffer<double>              partial, sums;

@@ -39,6 +39,9 @@ namespace ppf
     Runtime<PPF_APP> rt;
     std::string      error;
     bool             synchronize = true;
+    /// The adjoint driver of the application (created on first use), whose checkpoint
+    /// and segment buffers stay allocated for the following calls.
+    std::shared_ptr<void> adjoint;
   };
 
   inline std::string &
@@ -81,6 +84,16 @@ namespace ppf
       }
   }
 
+  template <class A>
+  auto &
+  adjoint_of(Context &c)
+  {
+    using Adjoint = AllenCahnAdjoint<A::dim, A::degree, typename A::FreeEnergy>;
+    if (!c.adjoint)
+      c.adjoint = std::make_shared<Adjoint>(c.rt.fine, c.rt.stream);
+    return *static_cast<Adjoint *>(c.adjoint.get());
+  }
+
   /// Adjoint entry points for applications that declare a free energy (the discarded
   /// branch must depend on a template parameter).
   template <class A>
@@ -91,8 +104,8 @@ namespace ppf
     if constexpr (has_free_energy<A>::value)
       {
         auto &rt = c.rt;
-        AllenCahnAdjoint<A::dim, A::degree, typename A::FreeEnergy> adj(rt.fine, rt.stream);
-        adj.forward(phi0, phi_final, rt.timer.get_timestep(), L, kappa, n, precision);
+        adjoint_of<A>(c).forward(phi0, phi_final, rt.timer.get_timestep(), L, kappa, n,
+                                 precision);
         rt.sync();
       }
     else
@@ -107,8 +120,8 @@ namespace ppf
     if constexpr (has_free_energy<A>::value)
       {
         auto &rt = c.rt;
-        AllenCahnAdjoint<A::dim, A::degree, typename A::FreeEnergy> adj(rt.fine, rt.stream);
-        adj.vjp(phi0, lam, rt.timer.get_timestep(), L, kappa, n, every, precision, dL, dk, dphi0);
+        adjoint_of<A>(c).vjp(phi0, lam, rt.timer.get_timestep(), L, kappa, n, every, precision,
+                             dL, dk, dphi0);
       }
     else
       throw Error(status_unsupported, "this application declares no free energy for the adjoint");

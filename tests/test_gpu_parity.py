@@ -299,12 +299,26 @@ def test_linear_initial_guess_option(device):
         ctx.set_option("linear_initial_guess", "random")
 
 
-ADJ_TEXT = prm.allen_cahn(dim=3, size=8.0, refinement=3, steps=0, dt=0.01, kappa=1.5)
+def _adj_text(dim, periodic, subdivisions=None):
+    text = prm.allen_cahn(dim=dim, size=8.0, refinement=3, steps=0, dt=0.01, kappa=1.5,
+                          subdivisions=subdivisions)
+    return text if periodic else text.replace("periodic = true", "periodic = false")
 
 
-@pytest.mark.parametrize("precision", [FP64])
-def test_adjoint_matches_oracle(device, precision):
-    ctx, orc = Context("allen_cahn_3d_q1", ADJ_TEXT), Oracle("allen_cahn_3d_q1", ADJ_TEXT)
+# Periodic and mirror-ghost (natural boundary) lattices, cubic and with unequal node
+# counts per axis (catches a transposed plane index), in 3D and 2D.
+ADJ_CASES = [
+    ("allen_cahn_3d_q1", _adj_text(3, True)),
+    ("allen_cahn_3d_q1", _adj_text(3, False, [2, 1, 3])),
+    ("allen_cahn_2d_q1", _adj_text(2, True, [3, 1])),
+    ("allen_cahn_2d_q1", _adj_text(2, False, [1, 2])),
+]
+
+
+@pytest.mark.parametrize("app,text", ADJ_CASES,
+                         ids=["3d_periodic", "3d_natural_2x1x3", "2d_periodic_3x1", "2d_natural_1x2"])
+def test_adjoint_matches_oracle(device, app, text, precision=FP64):
+    ctx, orc = Context(app, text), Oracle(app, text)
     info = ctx.info
     rng = np.random.default_rng(5)
     phi0 = rng.uniform(-1, 1, info.n_nodes)
