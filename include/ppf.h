@@ -100,7 +100,9 @@ int ppf_field_name(ppf_ctx *ctx, int field, char *buf, int len);
  * kernel, split_node: one thread per node); "stencil_waves" = 0|4|8,
  * "stencil_rows" = 0|1|2|4 (waves x rows: 4x2, 4x4, 8x1 or 8x2) and
  * "stencil_z_chunk" = <planes> (0: automatic) tune the tiled kernel's workgroup
- * shape; "synchronize" = 1|0 (0 leaves ppf_apply_operator asynchronous on the
+ * shape; "stencil_store_policy" = auto|temporal|nontemporal sets the cache policy of
+ * its output stores (auto: non-temporal when the launch's arrays exceed 192 MiB);
+ * "synchronize" = 1|0 (0 leaves ppf_apply_operator asynchronous on the
  * context's stream);
  * "mg_level_precision" = double|single, "mg_coarse_solver" = smoother|cg,
  * "linear_tolerance" = <value> and "linear_max_iterations" = <n> (applied to every

@@ -243,6 +243,18 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
       c.rt.stencil_z_chunk = std::stoi(v);
     else if (k == "stencil_target_blocks")
       c.rt.stencil_target_blocks = std::max(1, std::stoi(v));
+    else if (k == "stencil_store_policy")
+      {
+        if (v == "auto")
+          c.rt.stencil_store_policy = ppf::StorePolicy::Auto;
+        else if (v == "temporal")
+          c.rt.stencil_store_policy = ppf::StorePolicy::Temporal;
+        else if (v == "nontemporal")
+          c.rt.stencil_store_policy = ppf::StorePolicy::NonTemporal;
+        else
+          throw ppf::Error(ppf::status_invalid,
+                           "stencil_store_policy must be auto|temporal|nontemporal");
+      }
     else if (k == "synchronize")
       c.synchronize = v != "0";
     else if (k == "mg_level_precision")

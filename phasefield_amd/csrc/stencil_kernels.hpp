@@ -96,6 +96,8 @@ namespace ppf
     int           probe_tag;
     int           probe_n;
     const double *probe_ptr[PPF_MAX_FIELDS];
+    /// Output stores with the non-temporal cache policy (1) or the default one (0).
+    int           nt_store;
   };
 
   template <typename number>
@@ -188,14 +190,32 @@ namespace ppf
       return __builtin_bit_cast(number, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
   }
 
+  /// Output store; `nt` (uniform over the launch) selects the non-temporal cache
+  /// policy, bit 1 of the buffer instruction's cache-policy operand (NT on gfx950).
+  /// It keeps a launch's output from displacing its operand planes and halo rows in L2
+  /// and the Infinity Cache, which pays when the output will not be read from there
+  /// anyway (large lattices); a small lattice's output is read from the caches by the
+  /// next launch, and there the default policy is faster.
   template <typename number>
   __device__ PPF_INLINE void
-  buffer_store(number v, buffer_rsrc r, unsigned int voff, unsigned int soff)
+  buffer_store(number v, buffer_rsrc r, unsigned int voff, unsigned int soff, bool nt)
   {
     if constexpr (sizeof(number) == 8)
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+      {
+        if (nt)
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 2);
+        else
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+      }
     else
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
+      {
+        if (nt)
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff,
+                                                soff, 2);
+        else
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff,
+                                                soff, 0);
+      }
   }
 
   /// Operand s at lattice coordinate (x, y, z), which may lie a stencil radius outside
@@ -365,6 +385,7 @@ namespace ppf
     // Byte offsets fit 32 bits: the host runs this kernel on arrays below 4 GiB only.
     const unsigned int plane_bytes = unsigned(n0) * unsigned(n1) * unsigned(sizeof(number));
 
+    const bool  nt = a.nt_store != 0;
     buffer_rsrc r_op[NS], r_val[NF][PPF_N_SLOTS], r_dst[NF];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
@@ -622,10 +643,10 @@ namespace ppf
                   // dirichlet_faces_kernel, which writes the constraint values.
                   const bool dir = (fb & out.dirichlet_mask[f]) != 0;
                   if (valid && !(dir && out.dirichlet_write == dirichlet_value))
-                    buffer_store(dir ? number(0) : scale * acc, r_dst[f], own_b[r], zoff);
+                    buffer_store(dir ? number(0) : scale * acc, r_dst[f], own_b[r], zoff, nt);
                 }
               else if (valid)
-                buffer_store(scale * acc, r_dst[f], own_b[r], zoff);
+                buffer_store(scale * acc, r_dst[f], own_b[r], zoff, nt);
             }
           // Node 0 (lane 0, wave 0, row 0, plane 0 of workgroup 0, which owns tile 0 of
           // chunk 0) closes the increment's NaN probe.

@@ -121,10 +121,12 @@ def test_stencil_tilings_match_oracle(app, text, ops):
     info = ctx.info
     rng = np.random.default_rng(21)
     seed_fields(ctx, orc, rng)
-    for waves, rows, zc in TILINGS:
+    for i, (waves, rows, zc) in enumerate(TILINGS):
         ctx.set_option("stencil_waves", waves)
         ctx.set_option("stencil_rows", rows)
         ctx.set_option("stencil_z_chunk", zc)
+        # Both cache policies of the output stores, alternating over the tilings.
+        ctx.set_option("stencil_store_policy", ("temporal", "nontemporal")[i % 2])
         for block_id, which, nb in ops:
             for scaling in (SCALE_NONE, SCALE_INVM, SCALE_INVM_SQRT):
                 src_stored = rng.uniform(-1, 1, nb * info.n_nodes)
@@ -135,7 +137,7 @@ def test_stencil_tilings_match_oracle(app, text, ops):
                 ctx.apply_operator(block_id, which, scaling, FP64, d_src.data_ptr(),
                                    d_dst.data_ptr())
                 err = rel_err(d_dst.cpu().numpy(), block_to_stored(ref, info, nb))
-                assert err <= 1e-12, (waves, rows, zc, block_id, which, scaling, err)
+                assert err <= 1e-12, (waves, rows, zc, i % 2, block_id, which, scaling, err)
 
 
 @pytest.mark.parametrize("app,text,ops", OPERATOR_CASES[2:6], ids=[c[0] for c in OPERATOR_CASES[2:6]])
