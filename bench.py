@@ -20,7 +20,9 @@ Printed JSON line:
                conjugate-gradient + geometric-multigrid time to 1e-8 and 1e-10 relative
                residual, all-FP64 versus FP32 levels with FP64 outer iteration; Allen-Cahn
                3D adjoint: forward and vector-Jacobian-product times (FP64 and
-               FP32-forward) with their algorithmic GB/s (16 and 40 B per DoF and step).
+               FP32-forward) with their algorithmic GB/s (16 and 40 B per DoF and step);
+               implicit Allen-Cahn 2D (Newton blocks): time, Newton updates and CG steps
+               per increment, without and with GMG.
 """
 
 from __future__ import annotations
@@ -270,6 +272,43 @@ def adjoint_extra(torch):
     return res
 
 
+def newton_extra(torch):
+    """Implicit Allen-Cahn (Newton + CG per increment) at the reference application's
+    parameters (applications/allen_cahn/implicit/parameters.prm: 256^2 cells on
+    [0,100]^2, dt = 0.01, RMSEPerField tolerances 1e-6, no preconditioner), uniform mesh."""
+    from phasefield_amd import Context, prm
+
+    n_inc = 20
+    res = {"workload": "Allen-Cahn 2D implicit (Newton, CG), Q1, 256^2 cells on [0,100]^2, "
+                       f"dt=0.01, {n_inc} increments after 5 warm-up increments"}
+    for label, pc, depth in (("no_preconditioner", None, None), ("gmg", "GMG", 5)):
+        text = prm.allen_cahn_implicit(refinement=8, steps=n_inc + 6, outputs=0,
+                                       preconditioner=pc, mg_depth=depth)
+        ctx = Context("allen_cahn_implicit_2d_q1", text)
+        for _ in range(6):  # initial condition + 5 warm-up increments
+            ctx.solve_increment()
+        newton = cg = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n_inc):
+            if ctx.solve_increment() != 0:
+                raise RuntimeError("implicit Allen-Cahn increment stopped")
+            st = ctx.last_newton_stats(1)
+            newton += st.iterations
+            cg += st.linear_iterations
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[label] = {
+            "us_per_increment": dt / n_inc * 1e6,
+            "newton_iterations_per_increment": newton / n_inc,
+            "cg_iterations_per_increment": cg / n_inc,
+            "us_per_cg_iteration": dt / max(cg, 1) * 1e6,
+        }
+        del ctx
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -366,6 +405,7 @@ def main():
         extra = poisson_extra(torch)
         torch.cuda.empty_cache()
         extra["adjoint"] = adjoint_extra(torch)
+        extra["newton"] = newton_extra(torch)
     cpu = cpu_baseline(args.cpu_seconds) if args.cpu_seconds > 0 else None
 
     line = {

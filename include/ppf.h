@@ -160,6 +160,22 @@ int ppf_linear_solve(ppf_ctx *ctx, int block_id, ppf_solve_stats *stats);
 
 int ppf_last_solve_stats(ppf_ctx *ctx, int block_id, ppf_solve_stats *stats);
 
+/* Statistics of a Newton block's last solve (NewtonSolver::solve_impl,
+ * include/prismspf/solvers/newton_solver.h:73-156; the quantities of its log line,
+ * :140-147): Newton updates applied, CG steps over all of them, the l2 norm of the last
+ * residual evaluated and the tolerance (tolerance value times the linear solver's
+ * normalization).  ppf_last_solve_stats of the block reports its last linear solve. */
+typedef struct ppf_newton_stats
+{
+  int    iterations;
+  int    linear_iterations;
+  double residual;
+  double tolerance;
+  int    converged;
+} ppf_newton_stats;
+
+int ppf_last_newton_stats(ppf_ctx *ctx, int block_id, ppf_newton_stats *stats);
+
 /* Output diagnostics of problem.cc:517-549: l2-norm over deal.II's node set and the
  * integral sum(u_i * jxw_i). */
 int ppf_field_norms(ppf_ctx *ctx, int field, int age, double *l2, double *integral);

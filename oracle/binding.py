@@ -53,6 +53,9 @@ class Oracle:
             lib.oracle_linear_solve.argtypes = [
                 c_void_p, c_int, POINTER(c_int), _dp, _dp, _dp, POINTER(c_int),
             ]
+            lib.oracle_last_newton_stats.argtypes = [
+                c_void_p, c_int, POINTER(c_int), POINTER(c_int), _dp, _dp, POINTER(c_int),
+            ]
             lib.oracle_field_norms.argtypes = [c_void_p, c_int, c_int, _dp, _dp]
             lib.oracle_output_log.argtypes = [c_void_p]
             lib.oracle_output_log.restype = c_char_p
@@ -137,6 +140,15 @@ class Oracle:
             ctypes.byref(tol), ctypes.byref(conv)))
         return {"iterations": its.value, "initial_residual": r0.value,
                 "final_residual": r.value, "tolerance": tol.value, "converged": bool(conv.value)}
+
+    def last_newton_stats(self, block_id):
+        its, lin, conv = c_int(), c_int(), c_int()
+        r, tol = c_double(), c_double()
+        self._check(self.lib.oracle_last_newton_stats(
+            self.h, block_id, ctypes.byref(its), ctypes.byref(lin), ctypes.byref(r),
+            ctypes.byref(tol), ctypes.byref(conv)))
+        return {"iterations": its.value, "linear_iterations": lin.value, "residual": r.value,
+                "tolerance": tol.value, "converged": bool(conv.value)}
 
     def field_norms(self, f, age=0):
         l2, integral = c_double(), c_double()

@@ -23,6 +23,7 @@ BUILD = HERE / "build"
 
 APPS = [
     "allen_cahn_2d_q1",
+    "allen_cahn_implicit_2d_q1",
     "allen_cahn_3d_q1",
     "cahn_hilliard_3d_q2",
     "poisson_2d_q2",

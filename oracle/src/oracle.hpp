@@ -22,6 +22,8 @@
 //   * linear solve with the inhomogeneous-Dirichlet lift, deal.II SolverCG recurrences,
 //     Chebyshev (Lanczos eigenvalue estimate) and a geometric multigrid V-cycle
 //     (reference linear_solver.h:47-427);
+//   * the Newton loop of Newton blocks on the same operators and CG (reference
+//     newton_solver.h:73-156);
 //   * the explicit Allen-Cahn adjoint (no reference counterpart).
 // Cells are processed in 2^dim parity colours with OpenMP, so scatters never race.
 #pragma once
@@ -190,6 +192,9 @@ namespace oracle
           auto it = inputs.linear_solve_parameters.find(b.id);
           if (it != inputs.linear_solve_parameters.end())
             b.linear_solver_parameters = it->second;
+          const auto nit = inputs.nonlinear_solve_parameters.find(b.id);
+          if (nit != inputs.nonlinear_solve_parameters.end())
+            b.nonlinear_solver_parameters = nit->second;
         }
       for (auto &fa : fields)
         {
@@ -1187,26 +1192,16 @@ namespace oracle
     std::map<int, std::vector<double>> homog; // persistent initial guesses per block
     std::map<int, Stats>               last_stats;
 
+    /// do_linear_solve (reference linear_solver.h:348-402): the block's preconditioner,
+    /// set up for this solve, and deal.II SolverCG's recurrences from the initial guess x
+    /// (absolute residual tolerance = tolerance value * normalization).
     Stats
-    solve_linear(int b)
+    cg_solve(int b, std::vector<double> &x, const std::vector<double> &rhs)
     {
       const auto    bf = block_fields(b);
       const int64_t n  = int64_t(bf.size()) * top.ndofs;
       const int     sc = scaling_of(b);
       const auto   &lp = blocks[b].linear_solver_parameters;
-      std::vector<double> rhs(n), inh(n, 0.0), tmp(n);
-      apply_top(b, false, nullptr, rhs.data(), sc, false);
-      for (std::size_t k = 0; k < bf.size(); ++k)
-        {
-          const auto &c = cons[bf[k]];
-          for (int64_t i = 0; i < top.ndofs; ++i)
-            if (c.kind[i] == dirichlet)
-              inh[k * top.ndofs + i] = c.g[i];
-        }
-      apply_top(b, true, inh.data(), tmp.data(), sc, true);
-      for (int64_t i = 0; i < n; ++i)
-        rhs[i] -= tmp[i];
-
       Op A = [&](const std::vector<double> &x, std::vector<double> &y) {
         y.assign(x.size(), 0.0);
         apply_top(b, true, x.data(), y.data(), sc, false);
@@ -1233,9 +1228,6 @@ namespace oracle
           }
       };
 
-      auto &x = homog[b];
-      if (x.size() != std::size_t(n))
-        x.assign(n, 0.0);
       Stats st;
       st.tol = lp.tolerance * normalization(b);
       std::vector<double> g(n), h(n), d(n), ad(n);
@@ -1275,6 +1267,32 @@ namespace oracle
       st.iterations = int(step);
       st.r          = res;
       st.converged  = res <= st.tol;
+      return st;
+    }
+
+    Stats
+    solve_linear(int b)
+    {
+      const auto    bf = block_fields(b);
+      const int64_t n  = int64_t(bf.size()) * top.ndofs;
+      const int     sc = scaling_of(b);
+      std::vector<double> rhs(n), inh(n, 0.0), tmp(n);
+      apply_top(b, false, nullptr, rhs.data(), sc, false);
+      for (std::size_t k = 0; k < bf.size(); ++k)
+        {
+          const auto &c = cons[bf[k]];
+          for (int64_t i = 0; i < top.ndofs; ++i)
+            if (c.kind[i] == dirichlet)
+              inh[k * top.ndofs + i] = c.g[i];
+        }
+      apply_top(b, true, inh.data(), tmp.data(), sc, true);
+      for (int64_t i = 0; i < n; ++i)
+        rhs[i] -= tmp[i];
+
+      auto &x = homog[b];
+      if (x.size() != std::size_t(n))
+        x.assign(n, 0.0);
+      Stats st = cg_solve(b, x, rhs);
       for (std::size_t k = 0; k < bf.size(); ++k)
         {
           auto &sol = storage[bf[k]][0];
@@ -1284,6 +1302,66 @@ namespace oracle
         }
       last_stats[b] = st;
       return st;
+    }
+
+    struct NewtonStats
+    {
+      int    iterations = 0, linear_iterations = 0;
+      double residual = 0.0, tol = 0.0;
+      bool   converged = false;
+    };
+    std::map<int, std::vector<double>> newton_update; // persistent per block
+    std::map<int, NewtonStats>         last_newton;
+
+    /// NewtonSolver::solve_impl (reference newton_solver.h:73-156), step for step:
+    /// start from old_1 when the block's fields keep old states; each iteration applies
+    /// the constraints to the solution, evaluates the (scaled) rhs operator as the
+    /// residual, stops when its l2 norm is <= tolerance value * normalization, and
+    /// otherwise solves lhs(du) = r with CG from the previous update (the update vector
+    /// is a member of the solver and is never reset) and adds step_length * du.
+    NewtonStats
+    solve_newton(int b)
+    {
+      const auto    bf = block_fields(b);
+      const int64_t n  = int64_t(bf.size()) * top.ndofs;
+      const int     sc = scaling_of(b);
+      const auto   &np = blocks[b].nonlinear_solver_parameters;
+      NewtonStats   ns;
+      ns.tol = np.tolerance_value * normalization(b);
+      for (int f : bf)
+        if (storage[f].size() > 1)
+          storage[f][0] = storage[f][1];
+      auto &du = newton_update[b];
+      if (du.size() != std::size_t(n))
+        du.assign(n, 0.0);
+      std::vector<double> r(n);
+      bool                unconverged = true;
+      unsigned            iter        = 0;
+      while (unconverged && iter < np.max_iterations)
+        {
+          for (int f : bf)
+            distribute(cons[f], storage[f][0].data());
+          std::fill(r.begin(), r.end(), 0.0);
+          apply_top(b, false, nullptr, r.data(), sc, false);
+          ns.residual = std::sqrt(dot(r, r));
+          unconverged = ns.residual > ns.tol;
+          if (!unconverged)
+            continue;
+          const Stats st = cg_solve(b, du, r);
+          last_stats[b]  = st;
+          ns.linear_iterations += st.iterations;
+          for (std::size_t k = 0; k < bf.size(); ++k)
+            {
+              auto &sol = storage[bf[k]][0];
+              for (int64_t i = 0; i < top.ndofs; ++i)
+                sol[i] += np.step_length * du[k * top.ndofs + i];
+            }
+          ++iter;
+        }
+      ns.iterations  = int(iter);
+      ns.converged   = !unconverged;
+      last_newton[b] = ns;
+      return ns;
     }
 
     // ------------------------------------------------------------ time loop
@@ -1322,7 +1400,7 @@ namespace oracle
       else if (blk.solve_type == prismspf::Linear)
         solve_linear(b);
       else if (blk.solve_type == prismspf::Newton)
-        throw std::invalid_argument("Newton blocks are not part of the oracle");
+        solve_newton(b);
       if (timer.get_increment() == 0)
         for (int f : block_fields(b))
           for (std::size_t a = 1; a < storage[f].size(); ++a)

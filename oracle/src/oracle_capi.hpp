@@ -225,6 +225,22 @@ oracle_linear_solve(void *h, int block_id, int *its, double *r0, double *r, doub
 }
 
 int
+oracle_last_newton_stats(void *h, int block_id, int *its, int *lin_its, double *r, double *tol,
+                         int *converged)
+{
+  return oracle::guard(h, [&](auto &m) {
+    const int b  = m.block_index(block_id);
+    const auto st = m.last_newton.count(b) ? m.last_newton.at(b)
+                                           : typename std::decay_t<decltype(m)>::NewtonStats {};
+    *its       = st.iterations;
+    *lin_its   = st.linear_iterations;
+    *r         = st.residual;
+    *tol       = st.tol;
+    *converged = st.converged ? 1 : 0;
+  });
+}
+
+int
 oracle_field_norms(void *h, int f, int age, double *l2, double *integral)
 {
   return oracle::guard(h, [&](auto &m) {

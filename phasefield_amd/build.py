@@ -23,6 +23,7 @@ LIB_DIR = PKG / "lib"
 # application name -> source file (relative to the package)
 APPS = {
     "allen_cahn_2d_q1": "apps/allen_cahn/app_2d.hip",
+    "allen_cahn_implicit_2d_q1": "apps/allen_cahn_implicit/app_2d.hip",
     "allen_cahn_3d_q1": "apps/allen_cahn/app_3d.hip",
     "cahn_hilliard_3d_q2": "apps/cahn_hilliard/app.hip",
     "poisson_2d_q2": "apps/poisson/app_2d.hip",

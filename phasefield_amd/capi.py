@@ -44,6 +44,7 @@ EXPORTED_SYMBOLS = (
     "ppf_inverse_diagonal",
     "ppf_linear_solve",
     "ppf_last_solve_stats",
+    "ppf_last_newton_stats",
     "ppf_field_norms",
     "ppf_output_log",
     "ppf_stream",
@@ -85,6 +86,19 @@ class SolveStats(ctypes.Structure):
         ("iterations", c_int),
         ("initial_residual", c_double),
         ("final_residual", c_double),
+        ("tolerance", c_double),
+        ("converged", c_int),
+    ]
+
+
+class NewtonStats(ctypes.Structure):
+    """ppf_newton_stats: Newton updates, CG steps over all of them, last residual l2 norm,
+    tolerance, converged flag (reference newton_solver.h:140-147 logs the same)."""
+
+    _fields_ = [
+        ("iterations", c_int),
+        ("linear_iterations", c_int),
+        ("residual", c_double),
         ("tolerance", c_double),
         ("converged", c_int),
     ]
@@ -132,6 +146,7 @@ class Library:
         L.ppf_inverse_diagonal.argtypes = [vp, c_int, c_int, c_int, vp]
         L.ppf_linear_solve.argtypes = [vp, c_int, POINTER(SolveStats)]
         L.ppf_last_solve_stats.argtypes = [vp, c_int, POINTER(SolveStats)]
+        L.ppf_last_newton_stats.argtypes = [vp, c_int, POINTER(NewtonStats)]
         L.ppf_field_norms.argtypes = [vp, c_int, c_int, dp, dp]
         L.ppf_output_log.argtypes = [vp]
         L.ppf_output_log.restype = c_char_p
@@ -280,6 +295,11 @@ class Context:
     def last_solve_stats(self, block_id: int) -> SolveStats:
         st = SolveStats()
         self._check(self.L.ppf_last_solve_stats(self.handle, block_id, ctypes.byref(st)))
+        return st
+
+    def last_newton_stats(self, block_id: int) -> NewtonStats:
+        st = NewtonStats()
+        self._check(self.L.ppf_last_newton_stats(self.handle, block_id, ctypes.byref(st)))
         return st
 
     def field_norms(self, field: int, age: int = 0) -> tuple[float, float]:

@@ -445,6 +445,16 @@ ppf_last_solve_stats(ppf_ctx *ctx, int block_id, ppf_solve_stats *stats)
 }
 
 int
+ppf_last_newton_stats(ppf_ctx *ctx, int block_id, ppf_newton_stats *stats)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    const auto &st = c.rt.linear_state(c.rt.block_index(block_id)).newton;
+    if (stats)
+      *stats = {st.iterations, st.linear_iterations, st.residual, st.tolerance, st.converged};
+  });
+}
+
+int
 ppf_field_norms(ppf_ctx *ctx, int field, int age, double *l2, double *integral)
 {
   return ppf::guarded(ctx, [&](ppf::Context &c) {

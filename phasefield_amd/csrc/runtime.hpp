@@ -67,6 +67,17 @@ namespace ppf
     int    converged        = 0;
   };
 
+  /// Statistics of the last Newton solve of a block (the quantities of the reference's
+  /// log line, newton_solver.h:140-147).
+  struct NewtonStats
+  {
+    int    iterations        = 0;   ///< Newton updates applied
+    int    linear_iterations = 0;   ///< CG steps over all updates
+    double residual          = 0.0; ///< l2 norm of the last residual evaluated
+    double tolerance         = 0.0; ///< tolerance value * normalization
+    int    converged         = 0;
+  };
+
   template <int... ids, class F>
   void
   dispatch_block_id(std::integer_sequence<int, ids...>, int id, F &&f)
@@ -149,6 +160,9 @@ namespace ppf
           const auto it = inputs.linear_solve_parameters.find(b.id);
           if (it != inputs.linear_solve_parameters.end())
             b.linear_solver_parameters = it->second;
+          const auto nit = inputs.nonlinear_solve_parameters.find(b.id);
+          if (nit != inputs.nonlinear_solve_parameters.end())
+            b.nonlinear_solver_parameters = nit->second;
         }
       for (auto &fa : fields)
         {
@@ -1132,6 +1146,7 @@ namespace ppf
       DeviceBuffer<double> cheb_d, cheb_ax;
       std::vector<Level>   levels; // coarsest first
       SolveStats           stats;
+      NewtonStats          newton;
     };
 
     const prismspf::LinearSolverParameters &
@@ -1262,6 +1277,71 @@ namespace ppf
       sync();
       ls.stats = st;
       return st;
+    }
+
+    /// NewtonSolver::solve_impl (reference newton_solver.h:73-156).  The residual is
+    /// the block's rhs operator at the current solution and the Jacobian its lhs
+    /// operator, applied to the update (the "change" field) with homogeneous
+    /// constraints; both carry the linear solver's scaling.  The solution starts from
+    /// old_1 when the field keeps old states.  The update vector persists between
+    /// Newton iterations and increments: each linear solve starts from the previous
+    /// update, as the reference's newton_update member does.
+    NewtonStats
+    solve_newton(int b)
+    {
+      LinearState &ls = linear_state(b);
+      const auto  &bf = blocks[b].field_indices;
+      const auto  &np = blocks[b].nonlinear_solver_parameters;
+      const int    sc = linear_scaling(b);
+      NewtonStats  ns;
+      ns.tolerance = np.tolerance_value * normalization(b);
+
+      for (auto f : bf)
+        if (storage[f].size() > 1)
+          copy(field_ptr(int(f), 0), field_ptr(int(f), 1), fine.n_nodes);
+
+      bool     unconverged = true;
+      unsigned iter        = 0;
+      while (unconverged && iter < np.max_iterations)
+        {
+          // Constraints on the solution: Dirichlet values.
+          for (auto f : bf)
+            if (dirichlet_mask(int(f)))
+              hipLaunchKernelGGL((dirichlet_compose_kernel<dim, double>),
+                                 dim3(grid_for(fine.n_nodes, 256)), dim3(256), 0, stream,
+                                 field_ptr(int(f), 0), fine, dirichlet_values[f].get(),
+                                 dirichlet_mask(int(f)), 0.0, 1);
+          // Residual r = (scaled) rhs operator at the current solution.
+          apply_operator<double>(b, false, fine_fields(), {}, split_block(ls.rhs.get(), b, fine),
+                                 sc, dirichlet_zero, false);
+          ns.residual = std::sqrt(dot(ls.rhs.get(), ls.rhs.get(), ls.n));
+          if (!std::isfinite(ns.residual))
+            throw Error(status_nan, "Newton solver produced a non-finite residual");
+          unconverged = ns.residual > ns.tolerance;
+          if (!unconverged)
+            continue;
+          // J du = r, from the previous update (level operators see the current
+          // solution through the injected fields and re-evaluated diagonals).
+          prepare_preconditioner(b);
+          if (zero_initial_guess)
+            fill(ls.x_homog.get(), 0.0, ls.n);
+          ls.stats = cg(b, ls.x_homog.get(), ls.rhs.get(),
+                        lin_params(b).tolerance * normalization(b), lin_params(b).max_iterations);
+          ns.linear_iterations += ls.stats.iterations;
+          int k = 0;
+          for (auto f : bf)
+            {
+              axpby(field_ptr(int(f), 0), np.step_length, ls.x_homog.get() + k * fine.n_nodes,
+                    1.0, fine.n_nodes);
+              ++k;
+            }
+          ++iter;
+        }
+      sync();
+      ns.iterations = int(iter);
+      ns.converged  = unconverged ? 0 : 1;
+      ls.newton     = ns;
+      return ns;
     }
 
     /// Preconditioned CG with deal.II SolverCG's recurrences and stopping rule
@@ -1897,7 +1977,7 @@ namespace ppf
       else if (blk.solve_type == prismspf::Linear)
         solve_linear(b);
       else if (blk.solve_type == prismspf::Newton)
-        throw Error(status_unsupported, "Newton solve blocks are not supported yet");
+        solve_newton(b);
       if (timer.get_increment() == 0)
         copy_current_to_old(b);
       for (auto f : blocks[b].field_indices)

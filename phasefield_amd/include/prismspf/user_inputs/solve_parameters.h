@@ -52,11 +52,13 @@ struct LinearSolverParameters
   unsigned int   coarse_cg_max_iterations = 200;
 };
 
+/// Newton solver settings (reference include/prismspf/user_inputs/solve_parameters.h:142-175,
+/// defaults of src/user_inputs/solve_parameters.cc:352-400).
 struct NonlinearSolverParameters
 {
-  double       step_length    = 1.0;
-  unsigned int max_iterations = 100;
-  double       tolerance      = 1.0e-10;
+  double       step_length     = 1.0;
+  unsigned int max_iterations  = 100;
+  double       tolerance_value = 1.0e-10;
 };
 
 PRISMS_PF_END_NAMESPACE

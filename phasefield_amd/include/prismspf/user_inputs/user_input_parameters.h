@@ -12,6 +12,7 @@
 //   * `boundary conditions: k` {variables, conditions} (constraint_parameters.cc:11-106);
 //   * `linear solver parameters: k` (solve_parameters.cc:10-340) including the
 //     `Chebyshev` subsection;
+//   * `newton solver parameters: k` (solve_parameters.cc:352-470);
 //   * `output` {condition, number, list} (io_parameters.cc:100-150);
 //   * `random seed` (miscellaneous_parameters.cc:11-48);
 //   * `Model constant name = value, DOUBLE|INT|BOOL|TENSOR` (user_constants.h:960-1061).
@@ -314,6 +315,7 @@ public:
   UserConstants<dim>                              user_constants;
   std::map<std::string, BoundaryConditionSet>     boundary_parameters;
   std::map<int, LinearSolverParameters>           linear_solve_parameters;
+  std::map<int, NonlinearSolverParameters>        nonlinear_solve_parameters;
 
 private:
   void
@@ -469,6 +471,28 @@ private:
         lp.coarse_cg_tolerance = std::stod(f.get(p, {"mg coarse tolerance"}, "1.0e-12"));
         for (const auto &id : prm::split(f.get(p, {"solver_ids", "solver ids"}, ""), ','))
           linear_solve_parameters[std::stoi(id)] = lp;
+      }
+    // `newton solver parameters: k` (reference solve_parameters.cc:352-470).
+    for (unsigned int k = 0; k < 64; ++k)
+      {
+        const std::string p = "newton solver parameters: " + std::to_string(k) + "/";
+        if (!f.has_prefix(p))
+          continue;
+        NonlinearSolverParameters np;
+        np.max_iterations = std::stoul(f.get(p, {"max iterations"}, "100"));
+        if (np.max_iterations < 1)
+          throw std::invalid_argument("newton solver parameters: max iterations must be >= 1");
+        np.tolerance_value = std::stod(f.get(p, {"tolerance value", "tolerance"}, "1.0e-10"));
+        np.step_length     = std::stod(f.get(p, {"step size"}, "1.0"));
+        if (np.tolerance_value < 0.0 || np.step_length < 0.0)
+          throw std::invalid_argument(
+            "newton solver parameters: tolerance value and step size must be >= 0");
+        for (const auto &id : prm::split(f.get(p, {"solver_ids", "solve blocks", "solve_blocks",
+                                                   "solve block ids", "solve_block_ids",
+                                                   "solver ids"},
+                                               ""),
+                                         ','))
+          nonlinear_solve_parameters[std::stoi(id)] = np;
       }
   }
 };

@@ -63,6 +63,39 @@ def allen_cahn(dim=2, size=100.0, refinement=8, steps=1000, dt=0.005, L=1.0, kap
     ]) + "\n"
 
 
+def allen_cahn_implicit(size=100.0, refinement=8, steps=10000, dt=1.0e-2, m_well=1.0, kappa=2.0,
+                        linear_tolerance=1.0e-6, newton_tolerance=1.0e-6, newton_max_iterations=100,
+                        step_size=1.0, preconditioner=None, subdivisions=None, outputs=5,
+                        periodic=False, tolerance_type="RMSEPerField", mg_depth=None):
+    """The implicit Allen-Cahn application's parameters (defaults: the reference's
+    applications/allen_cahn/implicit/parameters.prm on a uniform mesh, 2-D)."""
+    lines = [
+        mesh_block(2, size, refinement, subdivisions, periodic=periodic),
+        _common(steps, dt, outputs),
+        f"set Model constant m_well = {m_well!r}, DOUBLE",
+        f"set Model constant kappa = {kappa!r}, DOUBLE",
+        "subsection linear solver parameters: 0",
+        "  set solver_ids = 1",
+        "  set max iterations = 1000",
+        f"  set tolerance type = {tolerance_type}",
+        f"  set tolerance value = {linear_tolerance!r}",
+    ]
+    if preconditioner:
+        lines.append(f"  set preconditioner type = {preconditioner}")
+        if mg_depth is not None:
+            lines.append(f"  set mg depth = {mg_depth}")
+    lines += [
+        "end",
+        "subsection newton solver parameters: 0",
+        "  set solver_ids = 1",
+        f"  set max iterations = {newton_max_iterations}",
+        f"  set step size = {step_size!r}",
+        f"  set tolerance value = {newton_tolerance!r}",
+        "end",
+    ]
+    return "\n".join(lines) + "\n"
+
+
 def cahn_hilliard(size=64.0, refinement=6, steps=200, dt=1e-4, M=1.0, kappa=1.0, outputs=1,
                   subdivisions=None):
     return "\n".join([

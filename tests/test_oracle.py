@@ -375,3 +375,43 @@ def test_adjoint_gradients_match_finite_differences(oracle_libs):
     assert abs(dL - fd_L) <= 1e-6 * abs(fd_L)
     assert abs(dk - fd_k) <= 1e-6 * abs(fd_k)
     assert abs(dphi @ v - fd_v) <= 1e-6 * abs(fd_v)
+
+
+@pytest.mark.parametrize("periodic", [False, True], ids=["natural", "periodic"])
+def test_newton_matches_dense_newton(oracle_libs, periodic):
+    """One implicit Allen-Cahn increment: the oracle's Newton loop against a dense Newton
+    on the brute-force assembled residual
+        R(n) = M (n_old - n - dt m f'(n)) - dt kappa m K n,  f'(n) = 4 n (n - 1)(n - 1/2),
+    with exact linear solves and the same stopping rule (RMSEPerField, one field:
+    ||M^-1/2 R||_2 <= tol sqrt(V), checked before each update, start from n_old).
+    Same iteration count, same root."""
+    dt, m, kappa, tol = 0.5, 1.0, 2.0, 1e-10  # 1 + dt m f'' > 0: SPD Jacobian for CG
+    text = prm.allen_cahn_implicit(refinement=3, steps=2, dt=dt, m_well=m, kappa=kappa,
+                                   linear_tolerance=1e-14, newton_tolerance=tol,
+                                   periodic=periodic)
+    o = oracle("allen_cahn_implicit_2d_q1", text)
+    o.solve_increment()  # increment 0: initial condition
+    n0 = o.get_field(0, -1)
+    o.solve_increment()
+    n1 = o.get_field(0, -1)
+    ns = o.last_newton_stats(1)
+
+    mesh = bf.Mesh(2, 1, [8, 8], [100.0, 100.0], [periodic, periodic])
+    K, mass = bf.assemble(mesh)
+    Kd = K.toarray()
+    u_old = n0[mesh.unique]
+
+    def residual(u):
+        return mass * (u_old - u - dt * m * 4.0 * u * (u - 1.0) * (u - 0.5)) - dt * kappa * m * (Kd @ u)
+
+    def jacobian(u):  # dR/du
+        return -np.diag(mass * (1.0 + dt * m * (12.0 * u * (u - 1.0) + 2.0))) - dt * kappa * m * Kd
+
+    u, its = u_old.copy(), 0
+    while np.linalg.norm(residual(u) / np.sqrt(mass)) > tol * 100.0 and its < 100:
+        u = u - np.linalg.solve(jacobian(u), residual(u))
+        its += 1
+    assert its >= 3  # a genuinely nonlinear step
+    assert ns["converged"] and ns["iterations"] == its
+    assert abs(ns["tolerance"] - tol * 100.0) <= 1e-15 * tol * 100.0
+    assert rel(n1[mesh.unique], u) < 1e-9
