@@ -22,7 +22,8 @@ Printed JSON line:
                3D adjoint: forward and vector-Jacobian-product times (FP64 and
                FP32-forward) with their algorithmic GB/s (16 and 40 B per DoF and step);
                implicit Allen-Cahn 2D (Newton blocks): time, Newton updates and CG steps
-               per increment, without and with GMG.
+               per increment, without and with GMG; implicit Cahn-Hilliard 2D (Newton
+               with BiCGStab) at the reference application's parameters.
 """
 
 from __future__ import annotations
@@ -306,6 +307,34 @@ def newton_extra(torch):
             "us_per_cg_iteration": dt / max(cg, 1) * 1e6,
         }
         del ctx
+    # Implicit Cahn-Hilliard at the reference application's parameters
+    # (applications/cahn_hilliard/implicit/parameters.prm: 128^2 Q2 cells on [0,100]^2,
+    # dt = 5e-4, BiCGStab with exact residual, tolerances 1e-6).
+    ctx = Context("cahn_hilliard_implicit_2d_q2",
+                  prm.cahn_hilliard_implicit(refinement=7, steps=n_inc + 6, outputs=0))
+    for _ in range(6):
+        ctx.solve_increment()
+    newton = its = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n_inc):
+        if ctx.solve_increment() != 0:
+            raise RuntimeError("implicit Cahn-Hilliard increment stopped")
+        st = ctx.last_newton_stats(0)
+        newton += st.iterations
+        its += st.linear_iterations
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res["cahn_hilliard_implicit"] = {
+        "workload": "Cahn-Hilliard 2D implicit (Newton, BiCGStab), Q2, 128^2 cells on [0,100]^2, "
+                    f"fields c and mu, dt=5e-4, {n_inc} increments after 5 warm-up increments",
+        "us_per_increment": dt / n_inc * 1e6,
+        "newton_iterations_per_increment": newton / n_inc,
+        "bicgstab_iterations_per_increment": its / n_inc,
+        "us_per_bicgstab_iteration": dt / max(its, 1) * 1e6,
+    }
+    del ctx
     return res
 
 

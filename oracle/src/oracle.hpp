@@ -39,6 +39,7 @@
 #include <cmath>
 #include <cstdint>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <set>
@@ -741,10 +742,23 @@ namespace oracle
     static double
     dot(const std::vector<double> &a, const std::vector<double> &b)
     {
+      // Fixed chunks summed in chunk order: the same result for any thread count and
+      // every run (an OpenMP reduction combines thread partials in arrival order).
+      constexpr int64_t   chunks = 64;
+      const int64_t       n      = int64_t(a.size());
+      std::vector<double> part(chunks, 0.0);
+#pragma omp parallel for schedule(static)
+      for (int64_t c = 0; c < chunks; ++c)
+        {
+          double        s  = 0.0;
+          const int64_t lo = n * c / chunks, hi = n * (c + 1) / chunks;
+          for (int64_t i = lo; i < hi; ++i)
+            s += a[i] * b[i];
+          part[c] = s;
+        }
       double s = 0.0;
-#pragma omp parallel for reduction(+ : s)
-      for (int64_t i = 0; i < int64_t(a.size()); ++i)
-        s += a[i] * b[i];
+      for (double v : part)
+        s += v;
       return s;
     }
 
@@ -1192,6 +1206,115 @@ namespace oracle
     std::map<int, std::vector<double>> homog; // persistent initial guesses per block
     std::map<int, Stats>               last_stats;
 
+    /// deal.II SolverBicgstab (deal.II >= 9.6, lac/solver_bicgstab.h: solve() and
+    /// iterate(); selected by `solver type = bicgstab`, reference linear_solver.h:461-490),
+    /// restated.  Convergence checks (SolverControl::check: success at res <= tol,
+    /// failure from step >= max on): at step 0 on |b - A x|, and in each step after the
+    /// half update on |r| (early success adds alpha y) and after the full update on
+    /// |A x - b| (exact residual) or |r|.  |r . rbar| or |rbar . v| below `breakdown`
+    /// restarts from the current x, keeping the step count.
+    template <class AOp, class POp>
+    static Stats
+    bicgstab(AOp &A, POp &P, std::vector<double> &x, const std::vector<double> &b, double tol,
+             unsigned max_steps, bool exact_residual, double breakdown)
+    {
+      const std::size_t   n = x.size();
+      std::vector<double> r(n), rbar(n), p(n), y(n), z(n), t(n), v(n);
+      Stats               st;
+      st.tol        = tol;
+      unsigned step = 0;
+      double   res  = 0.0;
+      bool     first = true;
+      auto     check = [&](double value) { return value <= tol ? 1 : (step >= max_steps ? -1 : 0); };
+      while (true)
+        {
+          A(x, r);
+          for (std::size_t i = 0; i < n; ++i)
+            r[i] = b[i] - r[i];
+          res = std::sqrt(dot(r, r));
+          if (first)
+            st.r0 = res;
+          first     = false;
+          int state = check(res);
+          if (state != 0)
+            {
+              st.converged = state > 0;
+              break;
+            }
+          double alpha = 1.0, omega = 1.0, rho = 1.0;
+          rbar         = r;
+          bool startup = true, restart = false;
+          do
+            {
+              ++step;
+              double rhobar = dot(r, rbar);
+              if (std::fabs(rhobar) < breakdown)
+                {
+                  restart = true;
+                  break;
+                }
+              const double beta = rhobar * alpha / (rho * omega);
+              rho               = rhobar;
+              if (startup)
+                {
+                  p       = r;
+                  startup = false;
+                }
+              else
+                for (std::size_t i = 0; i < n; ++i)
+                  p[i] = (beta * p[i] + r[i]) - beta * omega * v[i];
+              P(p, y);
+              A(y, v);
+              rhobar = dot(rbar, v);
+              if (std::fabs(rhobar) < breakdown)
+                {
+                  restart = true;
+                  break;
+                }
+              alpha = rho / rhobar;
+              for (std::size_t i = 0; i < n; ++i)
+                r[i] -= alpha * v[i];
+              res = std::sqrt(dot(r, r));
+              if (check(res) > 0)
+                {
+                  for (std::size_t i = 0; i < n; ++i)
+                    x[i] += alpha * y[i];
+                  state = 1;
+                  break;
+                }
+              P(r, z);
+              A(z, t);
+              rhobar            = dot(t, r);
+              const double t_sq = dot(t, t);
+              omega = t_sq < std::numeric_limits<double>::min() ? 0.0 : rhobar / t_sq;
+              for (std::size_t i = 0; i < n; ++i)
+                {
+                  x[i] += alpha * y[i] + omega * z[i];
+                  r[i] -= omega * t[i];
+                }
+              if (exact_residual)
+                {
+                  A(x, t);
+                  for (std::size_t i = 0; i < n; ++i)
+                    t[i] -= b[i];
+                  res = std::sqrt(dot(t, t));
+                }
+              else
+                res = std::sqrt(dot(r, r));
+              state = check(res);
+            }
+          while (state == 0);
+          if (!restart)
+            {
+              st.converged = state > 0;
+              break;
+            }
+        }
+      st.iterations = int(step);
+      st.r          = res;
+      return st;
+    }
+
     /// do_linear_solve (reference linear_solver.h:348-402): the block's preconditioner,
     /// set up for this solve, and deal.II SolverCG's recurrences from the initial guess x
     /// (absolute residual tolerance = tolerance value * normalization).
@@ -1230,6 +1353,12 @@ namespace oracle
 
       Stats st;
       st.tol = lp.tolerance * normalization(b);
+      if (lp.solver_type == "bicgstab")
+        return bicgstab(A, P, x, rhs, st.tol, lp.max_iterations, lp.bicgstab_exact_residual,
+                        lp.bicgstab_breakdown);
+      if (lp.solver_type != "cg")
+        throw std::invalid_argument("linear solver type " + lp.solver_type +
+                                    " is not part of the oracle (cg, bicgstab)");
       std::vector<double> g(n), h(n), d(n), ad(n);
       A(x, g);
       for (int64_t i = 0; i < n; ++i)

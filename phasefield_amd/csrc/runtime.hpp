@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -1144,6 +1145,7 @@ namespace ppf
       DeviceBuffer<double> inv_diag; // Chebyshev preconditioner on the fine level
       double               cheb_lmax = 0.0, cheb_lmin = 0.0;
       DeviceBuffer<double> cheb_d, cheb_ax;
+      DeviceBuffer<double> bs_r, bs_rbar, bs_p, bs_y, bs_z, bs_t, bs_v; // BiCGStab
       std::vector<Level>   levels; // coarsest first
       SolveStats           stats;
       NewtonStats          newton;
@@ -1262,8 +1264,9 @@ namespace ppf
       prepare_preconditioner(b);
       if (zero_initial_guess)
         fill(ls.x_homog.get(), 0.0, ls.n);
-      SolveStats st = cg(b, ls.x_homog.get(), ls.rhs.get(),
-                         lin_params(b).tolerance * normalization(b), lin_params(b).max_iterations);
+      SolveStats st = krylov(b, ls.x_homog.get(), ls.rhs.get(),
+                             lin_params(b).tolerance * normalization(b),
+                             lin_params(b).max_iterations);
 
       // solution = x + inhomogeneous values
       int k = 0;
@@ -1325,8 +1328,9 @@ namespace ppf
           prepare_preconditioner(b);
           if (zero_initial_guess)
             fill(ls.x_homog.get(), 0.0, ls.n);
-          ls.stats = cg(b, ls.x_homog.get(), ls.rhs.get(),
-                        lin_params(b).tolerance * normalization(b), lin_params(b).max_iterations);
+          ls.stats = krylov(b, ls.x_homog.get(), ls.rhs.get(),
+                            lin_params(b).tolerance * normalization(b),
+                            lin_params(b).max_iterations);
           ns.linear_iterations += ls.stats.iterations;
           int k = 0;
           for (auto f : bf)
@@ -1342,6 +1346,136 @@ namespace ppf
       ns.converged  = unconverged ? 0 : 1;
       ls.newton     = ns;
       return ns;
+    }
+
+    /// The block's linear solver (SolverSelector by `solver type`, reference
+    /// linear_solver.h:461-490): CG or BiCGStab on this path.
+    SolveStats
+    krylov(int b, double *x, const double *rhs, double tol, unsigned int max_steps)
+    {
+      const std::string &type = lin_params(b).solver_type;
+      if (type == "cg")
+        return cg(b, x, rhs, tol, max_steps);
+      if (type == "bicgstab")
+        return bicgstab(b, x, rhs, tol, max_steps);
+      throw Error(status_unsupported,
+                  "linear solver type '" + type + "' is not supported (cg, bicgstab)");
+    }
+
+    /// deal.II SolverBicgstab (deal.II >= 9.6, lac/solver_bicgstab.h; the reference
+    /// selects it by `solver type = bicgstab`), restated: residual r = b - A x checked at
+    /// step 0; each step checks |r| after the half update (early success adds alpha y)
+    /// and again after the full one, where `exact residual` recomputes |A x - b|; a
+    /// step whose |r . rbar| or |rbar . v| falls below `breakdown` restarts the
+    /// iteration from the current x with the step counter kept.
+    SolveStats
+    bicgstab(int b, double *x, const double *rhs, double tol, unsigned int max_steps)
+    {
+      LinearState &ls = linear_state(b);
+      const int64_t n = ls.n;
+      for (auto *buf : {&ls.bs_r, &ls.bs_rbar, &ls.bs_p, &ls.bs_y, &ls.bs_z, &ls.bs_t, &ls.bs_v})
+        buf->resize(n);
+      double *r = ls.bs_r.get(), *rbar = ls.bs_rbar.get(), *p = ls.bs_p.get(),
+             *y = ls.bs_y.get(), *z = ls.bs_z.get(), *t = ls.bs_t.get(), *v = ls.bs_v.get();
+      const auto  &lp        = lin_params(b);
+      const double breakdown = lp.bicgstab_breakdown;
+      SolveStats   st;
+      st.tolerance  = tol;
+      unsigned step = 0;
+      double   res  = 0.0;
+      bool     first = true, success = false;
+      // control.check: success at res <= tol, failure from step >= max_steps on.
+      auto check = [&](double value) {
+        if (!std::isfinite(value))
+          throw Error(status_nan, "linear solver produced a non-finite residual");
+        if (value <= tol)
+          return 1;
+        return step >= max_steps ? -1 : 0;
+      };
+      while (true) // one pass per (re)start
+        {
+          vmult_fine(b, r, x);
+          axpby(r, 1.0, rhs, -1.0, n); // r = b - A x
+          res = std::sqrt(dot(r, r, n));
+          if (first)
+            st.initial_residual = res;
+          first     = false;
+          int state = check(res);
+          if (state != 0)
+            {
+              success = state > 0;
+              break;
+            }
+          double alpha = 1.0, omega = 1.0, rho = 1.0;
+          copy(rbar, r, n);
+          bool startup = true, restart = false;
+          do
+            {
+              ++step;
+              double rhobar = dot(r, rbar, n);
+              if (std::fabs(rhobar) < breakdown)
+                {
+                  restart = true;
+                  break;
+                }
+              const double beta = rhobar * alpha / (rho * omega);
+              rho               = rhobar;
+              if (startup)
+                {
+                  copy(p, r, n);
+                  startup = false;
+                }
+              else
+                {
+                  axpby(p, 1.0, r, beta, n);          // p = beta p + r
+                  axpby(p, -beta * omega, v, 1.0, n); // p -= beta omega v
+                }
+              precondition(b, y, p);
+              vmult_fine(b, v, y);
+              rhobar = dot(rbar, v, n);
+              if (std::fabs(rhobar) < breakdown)
+                {
+                  restart = true;
+                  break;
+                }
+              alpha = rho / rhobar;
+              axpby(r, -alpha, v, 1.0, n);
+              res = std::sqrt(dot(r, r, n));
+              if (check(res) > 0)
+                {
+                  axpby(x, alpha, y, 1.0, n);
+                  state = 1;
+                  break;
+                }
+              precondition(b, z, r);
+              vmult_fine(b, t, z);
+              rhobar             = dot(t, r, n);
+              const double t_sq  = dot(t, t, n);
+              omega              = t_sq < std::numeric_limits<double>::min() ? 0.0 : rhobar / t_sq;
+              axpby(x, alpha, y, 1.0, n);
+              axpby(x, omega, z, 1.0, n);
+              axpby(r, -omega, t, 1.0, n);
+              if (lp.bicgstab_exact_residual)
+                {
+                  vmult_fine(b, t, x);
+                  axpby(t, -1.0, rhs, 1.0, n); // t = A x - b
+                  res = std::sqrt(dot(t, t, n));
+                }
+              else
+                res = std::sqrt(dot(r, r, n));
+              state = check(res);
+            }
+          while (state == 0);
+          if (!restart)
+            {
+              success = state > 0;
+              break;
+            }
+        }
+      st.iterations     = int(step);
+      st.final_residual = res;
+      st.converged      = success ? 1 : 0;
+      return st;
     }
 
     /// Preconditioned CG with deal.II SolverCG's recurrences and stopping rule

@@ -44,6 +44,12 @@ struct LinearSolverParameters
   unsigned int        mg_depth       = 1;
   PreconditionerType  preconditioner = None;
   ChebyshevParameters chebyshev_parameters;
+  /// deal.II SolverBicgstab::AdditionalData as the reference fills it
+  /// (src/user_inputs/solve_parameters.cc:213-230).  The declared default of
+  /// "breakdown" is std::to_string(numeric_limits<double>::min()), i.e. the text
+  /// "0.000000", so the threshold is 0 unless a parameter file sets it.
+  bool   bicgstab_exact_residual = true;
+  double bicgstab_breakdown      = 0.0;
 
   // Settings of the MI355X path with no reference counterpart.
   LevelPrecision level_precision        = LevelPrecision::Double;

@@ -11,7 +11,7 @@
 //     (temporal_discretization.cc:35-107);
 //   * `boundary conditions: k` {variables, conditions} (constraint_parameters.cc:11-106);
 //   * `linear solver parameters: k` (solve_parameters.cc:10-340) including the
-//     `Chebyshev` subsection;
+//     `Chebyshev` and `BiCGStab` subsections;
 //   * `newton solver parameters: k` (solve_parameters.cc:352-470);
 //   * `output` {condition, number, list} (io_parameters.cc:100-150);
 //   * `random seed` (miscellaneous_parameters.cc:11-48);
@@ -461,6 +461,13 @@ private:
           std::stod(f.get(c, {"smoothing range"}, "15.0"));
         lp.chebyshev_parameters.eig_cg_n_iterations =
           std::stoul(f.get(c, {"eigenvalue cg iterations"}, "10"));
+        const std::string bs = p + "BiCGStab/";
+        lp.bicgstab_exact_residual = prm::to_bool(f.get(bs, {"exact residual"}, "true"));
+        lp.bicgstab_breakdown      = std::stod(f.get(bs, {"breakdown"}, "0.000000"));
+        static const std::set<std::string> solver_names = {
+          "richardson", "cg", "bicgstab", "gmres", "fgmres", "minres"};
+        if (!solver_names.count(lp.solver_type))
+          throw std::invalid_argument("Invalid solver type: " + lp.solver_type);
         // MI355X-path settings (no reference counterpart).
         const std::string lvl = f.get(p, {"mg level precision"}, "double");
         lp.level_precision =

@@ -96,6 +96,48 @@ def allen_cahn_implicit(size=100.0, refinement=8, steps=10000, dt=1.0e-2, m_well
     return "\n".join(lines) + "\n"
 
 
+def cahn_hilliard_implicit(size=100.0, refinement=7, steps=40000, dt=5.0e-4, McV=1.0, KcV=1.5,
+                           WcV=16.0, ic_type=0, c0=0.5, icamplitude=0.1, solver="bicgstab",
+                           linear_tolerance=1.0e-6, newton_tolerance=1.0e-6,
+                           newton_max_iterations=100, exact_residual=True, preconditioner=None,
+                           subdivisions=None, outputs=500, periodic=False,
+                           tolerance_type="RMSEPerField"):
+    """The implicit Cahn-Hilliard application's parameters (defaults: the reference's
+    applications/cahn_hilliard/implicit/parameters.prm on a uniform mesh, 2-D; end time
+    20 = 40000 steps of 5e-4)."""
+    lines = [
+        mesh_block(2, size, refinement, subdivisions, periodic=periodic),
+        _common(steps, dt, outputs),
+        f"set Model constant McV = {McV!r}, DOUBLE",
+        f"set Model constant KcV = {KcV!r}, DOUBLE",
+        f"set Model constant WcV = {WcV!r}, DOUBLE",
+        f"set Model constant ic_type = {ic_type}, INT",
+        f"set Model constant c0 = {c0!r}, DOUBLE",
+        f"set Model constant icamplitude = {icamplitude!r}, DOUBLE",
+        "subsection linear solver parameters: 0",
+        "  set solver_ids = 0",
+        f"  set solver type = {solver}",
+        "  set max iterations = 10000",
+        f"  set tolerance type = {tolerance_type}",
+        f"  set tolerance value = {linear_tolerance!r}",
+    ]
+    if preconditioner:
+        lines.append(f"  set preconditioner type = {preconditioner}")
+    lines += [
+        "  subsection BiCGStab",
+        f"    set exact residual = {'true' if exact_residual else 'false'}",
+        "  end",
+        "end",
+        "subsection newton solver parameters: 0",
+        "  set solver_ids = 0",
+        f"  set max iterations = {newton_max_iterations}",
+        "  set step size = 1.0",
+        f"  set tolerance value = {newton_tolerance!r}",
+        "end",
+    ]
+    return "\n".join(lines) + "\n"
+
+
 def cahn_hilliard(size=64.0, refinement=6, steps=200, dt=1e-4, M=1.0, kappa=1.0, outputs=1,
                   subdivisions=None):
     return "\n".join([

@@ -1,0 +1,75 @@
+"""Newton blocks with BiCGStab on the GPU against the oracle (implicit Cahn-Hilliard
+application: two fields, non-symmetric Jacobian, Q2, 2-D).
+
+BiCGStab is deal.II's SolverBicgstab as the reference selects it (`solver type =
+bicgstab`, reference include/prismspf/solvers/linear_solver.h:461-490), restated on
+both sides.  Per increment: the same number of Newton updates, BiCGStab steps within
+15 % (see CASES), the new (c, mu) and the post-processed free-energy density within 1e-9
+relative at tight solver tolerances.  Cases: exact and recurrence residuals, no preconditioner and Chebyshev,
+natural and periodic boundaries, the reference's own parameters (dt = 5e-4, tolerances
+1e-6) on a 129 x 129-node lattice whose operators run on the tiled stencil kernel, and
+a solver type this path does not implement (an error, not a fallback).
+"""
+
+from __future__ import annotations
+
+import pytest
+
+from phasefield_amd import Context, PPFError, prm
+from phasefield_amd.capi import AGE_LATEST
+from oracle.binding import Oracle
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+APP = "cahn_hilliard_implicit_2d_q2"
+
+# name -> (prm text, field tolerance).  BiCGStab's residual history is irregular, so
+# the step at which it first meets its tolerance moves with the last bits of its dot
+# products, which the oracle and the GPU sum in different orders (changing only the
+# oracle's summation order moved one of these solves from 74 to 81 steps).  Step counts
+# are compared within 15 %.  At tight solver tolerances the fields still agree to
+# rounding; at the reference's tolerances (1e-6 on the RMSE-scaled residuals) the
+# solution is fixed only to those tolerances, and the fields differ by ~1e-5.
+CASES = {
+    "exact_residual": (prm.cahn_hilliard_implicit(refinement=5, steps=3, dt=1e-3,
+                                                  linear_tolerance=1e-12,
+                                                  newton_tolerance=1e-10), 1e-9),
+    "recurrence_residual_periodic": (prm.cahn_hilliard_implicit(
+        refinement=5, steps=3, dt=1e-3, linear_tolerance=1e-12, newton_tolerance=1e-10,
+        exact_residual=False, periodic=True), 1e-9),
+    "chebyshev_grains": (prm.cahn_hilliard_implicit(refinement=5, steps=3, dt=0.01, ic_type=1,
+                                                    linear_tolerance=1e-12,
+                                                    newton_tolerance=1e-10,
+                                                    preconditioner="Chebyshev"), 1e-9),
+    "reference_parameters_129": (prm.cahn_hilliard_implicit(refinement=6, steps=3), 5e-4),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_newton_bicgstab_matches_oracle(device, name):
+    text, field_tol = CASES[name]
+    ctx, orc = Context(APP, text), Oracle(APP, text)
+    assert ctx.solve_increment() == 0 and orc.solve_increment() == 0  # initial condition
+    for inc in range(1, ctx.info.n_increments + 1):
+        assert ctx.solve_increment() == 0
+        assert orc.solve_increment() == 0
+        g, o = ctx.last_newton_stats(0), orc.last_newton_stats(0)
+        print(f"{name} increment {inc}: newton {g.iterations}/{o['iterations']}, "
+              f"bicgstab {g.linear_iterations}/{o['linear_iterations']}, "
+              f"residual {g.residual:.3e}/{o['residual']:.3e}", flush=True)
+        assert g.iterations == o["iterations"] and g.converged and o["converged"]
+        assert g.residual <= g.tolerance
+        slack = max(g.iterations, (15 * o["linear_iterations"]) // 100)
+        assert abs(g.linear_iterations - o["linear_iterations"]) <= slack
+        for f in range(3):
+            err = rel_err(ctx.get_field(f, AGE_LATEST).ravel(), orc.get_field(f, AGE_LATEST))
+            assert err <= field_tol, (name, inc, f, err)
+
+
+def test_unsupported_solver_type_fails_loudly(device):
+    text = prm.cahn_hilliard_implicit(size=25.0, refinement=3, steps=1, solver="gmres")
+    ctx = Context(APP, text)
+    ctx.solve_increment()
+    with pytest.raises(PPFError, match="gmres"):
+        ctx.solve_increment()

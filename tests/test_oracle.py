@@ -415,3 +415,70 @@ def test_newton_matches_dense_newton(oracle_libs, periodic):
     assert ns["converged"] and ns["iterations"] == its
     assert abs(ns["tolerance"] - tol * 100.0) <= 1e-15 * tol * 100.0
     assert rel(n1[mesh.unique], u) < 1e-9
+
+
+def test_newton_bicgstab_matches_dense_newton_and_scipy(oracle_libs):
+    """Implicit Cahn-Hilliard increment (two fields, non-symmetric Jacobian, Q2): the
+    oracle's Newton loop with its BiCGStab restatement against
+
+    * a dense Newton on the brute-force assembled residual
+        R_c  = M (c_old - c) - dt Mc K mu,
+        R_mu = M (mu - W c (c - 1)(c - 1/2)) - Kc K c,
+      with exact linear solves and the same stopping rule (RMSEPerField, two fields:
+      ||M^-1/2 R||_2 <= tol sqrt(V) sqrt(2)): same iteration count, same root;
+    * scipy's BiCGStab on the same scaled Newton systems M^-1/2 J du = M^-1/2 R, each
+      started from the previous update as the Newton solver does (recurrence residual,
+      i.e. `exact residual = false`, no preconditioner): total BiCGStab steps within one
+      per Newton update plus 5 %.
+    """
+    dt, Mc, Kc, W, tol, lin_tol = 0.05, 1.0, 1.5, 16.0, 1e-10, 1e-12
+    size, ref = 16.0, 2
+    text = prm.cahn_hilliard_implicit(size=size, refinement=ref, steps=2, dt=dt, McV=Mc, KcV=Kc,
+                                      WcV=W, ic_type=0, icamplitude=0.2, linear_tolerance=lin_tol,
+                                      newton_tolerance=tol, exact_residual=False)
+    o = oracle("cahn_hilliard_implicit_2d_q2", text)
+    o.solve_increment()
+    c0 = o.get_field(0, -1)
+    o.solve_increment()
+    c1, mu1 = o.get_field(0, -1), o.get_field(1, 0)
+    ns = o.last_newton_stats(0)
+
+    mesh = bf.Mesh(2, 2, [4, 4], [size, size], [False, False])
+    K, mass = bf.assemble(mesh)
+    Kd = K.toarray()
+    nu = mass.size
+    cold = c0[mesh.unique]
+    Dinv = np.concatenate([1.0 / np.sqrt(mass)] * 2)
+
+    def residual(x):
+        c, mu = x[:nu], x[nu:]
+        return np.concatenate([mass * (cold - c) - dt * Mc * (Kd @ mu),
+                               mass * (mu - W * c * (c - 1.0) * (c - 0.5)) - Kc * (Kd @ c)])
+
+    def lhs(x):  # the Jacobian operator the lhs hooks assemble (= -dR/dx)
+        c = x[:nu]
+        fp = W * (0.5 + 3.0 * c * (c - 1.0))
+        return np.block([[np.diag(mass), dt * Mc * Kd],
+                         [np.diag(mass * fp) + Kc * Kd, -np.diag(mass)]])
+
+    norm = tol * size * np.sqrt(2.0)
+    x = np.concatenate([cold, np.zeros(nu)])  # mu starts from its initial value 0
+    du = np.zeros(2 * nu)
+    its, scipy_steps = 0, 0
+    while np.linalg.norm(Dinv * residual(x)) > norm and its < 50:
+        A, b = Dinv[:, None] * lhs(x), Dinv * residual(x)
+        count = [0]
+        du, info = spla.bicgstab(A, b, x0=du, rtol=0.0, atol=lin_tol * size * np.sqrt(2.0),
+                                 maxiter=10000, callback=lambda xk: count.__setitem__(0, count[0] + 1))
+        assert info == 0
+        scipy_steps += count[0]
+        x = x + np.linalg.solve(lhs(x), residual(x))  # exact Newton update
+        its += 1
+    assert its >= 2
+    assert ns["converged"] and ns["iterations"] == its
+    assert rel(c1[mesh.unique], x[:nu]) < 1e-9
+    assert rel(mu1[mesh.unique], x[nu:]) < 1e-9
+    # scipy counts a step that ends at the half-update test differently (one per solve),
+    # and a BiCGStab step count at a tight tolerance moves by a few with the last bits of
+    # its dot products: 5 % slack on top.
+    assert abs(ns["linear_iterations"] - scipy_steps) <= its + scipy_steps // 20, (ns, scipy_steps)
