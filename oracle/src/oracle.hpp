@@ -1572,11 +1572,25 @@ namespace oracle
       return {std::sqrt(s2), si};
     }
 
+    /// ConstraintManager::update_time_dependent_constraints (reference
+    /// constraint_manager.cc:380-423, called first in problem.cc:428-430): the
+    /// constraints of fields whose boundary conditions are time dependent are rebuilt
+    /// from set_dirichlet at the current time.  Level constraints are homogeneous and
+    /// do not change.
+    void
+    update_time_dependent_constraints()
+    {
+      for (std::size_t f = 0; f < fields.size(); ++f)
+        if (fields[f].boundary_conditions.time_dependent)
+          cons[f] = make_constraints(top, int(f), false);
+    }
+
     int
     solve_increment()
     {
       const unsigned inc = timer.get_increment();
       const bool     out = inputs.output_parameters.should_output(inc);
+      update_time_dependent_constraints();
       for (std::size_t b = 0; b < blocks.size(); ++b)
         {
           const auto t = blocks[b].solve_timing;

@@ -54,7 +54,9 @@ private:
                 [[maybe_unused]] number                   &scalar_value,
                 [[maybe_unused]] number &vector_component_value) const override
   {
-    scalar_value = number(0.25 * (boundary_id + 1) + 0.1 * point[0]);
+    // The time term changes the values only where the boundary conditions are declared
+    // time dependent (they are otherwise evaluated once, at time 0).
+    scalar_value = number(0.25 * (boundary_id + 1) + 0.1 * point[0] + 2.0 * sim_timer.get_time());
   }
 
   void

@@ -290,6 +290,8 @@ namespace ppf
     {
       out.clear();
       out.resize(fields.size());
+      dirichlet_nodes.assign(fields.size(), {});
+      dirichlet_bids.assign(fields.size(), {});
       for (std::size_t f = 0; f < fields.size(); ++f)
         {
           const unsigned int mask = dirichlet_mask(int(f));
@@ -305,12 +307,9 @@ namespace ppf
               for (unsigned int bid = 0; bid < 2u * dim; ++bid)
                 if (on_masked_face(g, int(bid / 2), i[bid / 2], mask & (1u << bid)))
                   {
-                    dealii::Point<dim> p;
-                    for (int d = 0; d < dim; ++d)
-                      p[d] = g.lower[d] + i[d] * g.h[d] / degree;
-                    double value = 0.0, vec = 0.0;
-                    pde_d->set_dirichlet(unsigned(f), bid, 0u, p, timer, value, vec);
-                    host[node] = value;
+                    host[node] = dirichlet_at(int(f), bid, i);
+                    dirichlet_nodes[f].push_back(node);
+                    dirichlet_bids[f].push_back(bid);
                     break;
                   }
             }
@@ -318,6 +317,77 @@ namespace ppf
           PPF_HIP_CHECK(hipMemcpyAsync(out[f].get(), host.data(), sizeof(double) * g.n_nodes,
                                        hipMemcpyHostToDevice, stream));
         }
+      sync();
+    }
+
+    /// set_dirichlet of field f at lattice node i (boundary id bid) at the current time.
+    double
+    dirichlet_at(int f, unsigned int bid, const int (&i)[3])
+    {
+      dealii::Point<dim> p;
+      for (int d = 0; d < dim; ++d)
+        p[d] = fine.lower[d] + i[d] * fine.h[d] / degree;
+      double value = 0.0, vec = 0.0;
+      pde_d->set_dirichlet(unsigned(f), bid, 0u, p, timer, value, vec);
+      return value;
+    }
+
+    bool
+    any_time_dependent_dirichlet() const
+    {
+      for (std::size_t f = 0; f < fields.size(); ++f)
+        if (fields[f].boundary_conditions.time_dependent && !dirichlet_nodes[f].empty())
+          return true;
+      return false;
+    }
+
+    /// ConstraintManager::update_time_dependent_constraints (reference
+    /// constraint_manager.cc:380-423, called first in every increment, problem.cc:
+    /// 428-430): the Dirichlet values of fields whose boundary conditions are time
+    /// dependent are re-evaluated at the current time, on their boundary nodes only,
+    /// and scattered into the device arrays (and their FP32 copies).  Multigrid levels
+    /// use homogeneous constraints and do not change.
+    void
+    update_time_dependent_dirichlet()
+    {
+      dirichlet_idx.resize(fields.size());
+      dirichlet_stage.resize(fields.size());
+      dirichlet_host.resize(fields.size());
+      for (std::size_t f = 0; f < fields.size(); ++f)
+        {
+          const auto &nodes = dirichlet_nodes[f];
+          if (!fields[f].boundary_conditions.time_dependent || nodes.empty())
+            continue;
+          const int64_t count = int64_t(nodes.size());
+          auto         &host  = dirichlet_host[f];
+          host.resize(nodes.size());
+          for (int64_t k = 0; k < count; ++k)
+            {
+              const int64_t node = nodes[k];
+              int           i[3];
+              i[0]    = int(node % fine.nodes[0]);
+              i[1]    = int((node / fine.nodes[0]) % fine.nodes[1]);
+              i[2]    = dim == 3 ? int(node / (int64_t(fine.nodes[0]) * fine.nodes[1])) : 0;
+              host[k] = dirichlet_at(int(f), dirichlet_bids[f][k], i);
+            }
+          if (dirichlet_idx[f].size() != count)
+            {
+              dirichlet_idx[f].resize(count);
+              PPF_HIP_CHECK(hipMemcpyAsync(dirichlet_idx[f].get(), nodes.data(),
+                                           sizeof(int64_t) * count, hipMemcpyHostToDevice,
+                                           stream));
+            }
+          dirichlet_stage[f].resize(count);
+          PPF_HIP_CHECK(hipMemcpyAsync(dirichlet_stage[f].get(), host.data(),
+                                       sizeof(double) * count, hipMemcpyHostToDevice, stream));
+          hipLaunchKernelGGL(scatter_kernel<double>, dim3(grid_for(count, 256)), dim3(256), 0,
+                             stream, dirichlet_values[f].get(), dirichlet_idx[f].get(),
+                             dirichlet_stage[f].get(), count);
+          PPF_HIP_CHECK(hipGetLastError());
+          if (f < dirichlet_values_f.size() && dirichlet_values_f[f].get())
+            convert(dirichlet_values_f[f].get(), dirichlet_values[f].get(), fine.n_nodes);
+        }
+      // The host staging vectors are reused by the next increment.
       sync();
     }
 
@@ -2169,6 +2239,8 @@ namespace ppf
     enqueue_increment(bool do_output, int slot, int *halt)
     {
       const unsigned int inc = timer.get_increment();
+      if (any_time_dependent_dirichlet())
+        update_time_dependent_dirichlet();
       std::vector<int>   solve;
       for (std::size_t b = 0; b < blocks.size(); ++b)
         {
@@ -2261,7 +2333,7 @@ namespace ppf
     speculable(unsigned int inc) const
     {
       if (inc == 0 || inputs.output_parameters.should_output(inc) || pde_d->get_user_stop() ||
-          explicit_fp32)
+          explicit_fp32 || any_time_dependent_dirichlet())
         return false;
       for (const auto &blk : blocks)
         {
@@ -2425,6 +2497,13 @@ namespace ppf
 
     std::vector<std::vector<DeviceBuffer<double>>> storage;
     std::vector<DeviceBuffer<double>>              dirichlet_values;
+    /// Boundary nodes (and the boundary id that set each) of every field's Dirichlet
+    /// faces, for re-evaluating time-dependent values.
+    std::vector<std::vector<int64_t>>              dirichlet_nodes;
+    std::vector<std::vector<unsigned int>>         dirichlet_bids;
+    std::vector<DeviceBuffer<int64_t>>             dirichlet_idx;
+    std::vector<DeviceBuffer<double>>              dirichlet_stage;
+    std::vector<std::vector<double>>               dirichlet_host;
     std::map<int, LinearState>                     linear_states;
     std::map<OpKey, PlanEntry>                     plans;
     DeviceBuffer<double>                           scratch_d;

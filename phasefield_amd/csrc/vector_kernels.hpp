@@ -363,6 +363,17 @@ namespace ppf
       }
   }
 
+  /// dst[idx[k]] = vals[k] for k < count.
+  template <typename T>
+  __global__ void
+  scatter_kernel(T *__restrict__ dst, const int64_t *__restrict__ idx, const T *__restrict__ vals,
+                 int64_t count)
+  {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < count)
+      dst[idx[k]] = vals[k];
+  }
+
   /// dst = value at non-Dirichlet nodes, g at Dirichlet nodes.
   template <int dim, typename T>
   __global__ void

@@ -197,7 +197,8 @@ def precipitate(size=192.0, refinement=6, subdivisions=(3, 3, 3), steps=1000, dt
 
 
 def testbed(dim=2, refinement=3, size=4.0, steps=2, dt=1e-3, dirichlet_faces=(0, 3),
-            tolerance=1e-10, preconditioner="None", mg_depth=2, periodic=False):
+            tolerance=1e-10, preconditioner="None", mg_depth=2, periodic=False,
+            time_dependent=False):
     conds = ["Natural"] * (2 * dim)
     for f in dirichlet_faces:
         conds[f] = "Dirichlet"
@@ -207,6 +208,7 @@ def testbed(dim=2, refinement=3, size=4.0, steps=2, dt=1e-3, dirichlet_faces=(0,
         "subsection boundary conditions: 0",
         "  set variables = c, v",
         f"  set conditions = {', '.join(conds)}",
+        f"  set time dependent = {'true' if time_dependent else 'false'}",
         "end",
         "subsection linear solver parameters: 0",
         "  set solver_ids = 1",

@@ -182,10 +182,17 @@ RUN_CASES = [
     ("precipitate_3d_q1", prm.precipitate(size=16.0, refinement=3, subdivisions=(1, 1, 1), steps=10, outputs=2), 10),
     ("testbed_2d_q2", prm.testbed(dim=2, refinement=3, steps=5), 5),
     ("testbed_3d_q1", prm.testbed(dim=3, refinement=2, steps=3), 3),
+    # Time-dependent Dirichlet values (explicit c and linear v re-constrained every
+    # increment, reference constraint_manager.cc:380-423).
+    ("testbed_2d_q2", prm.testbed(dim=2, refinement=3, steps=5, dt=0.01, time_dependent=True), 5),
+    ("testbed_3d_q1", prm.testbed(dim=3, refinement=2, steps=3, dt=0.01, dirichlet_faces=(1, 4),
+                                  time_dependent=True), 3),
 ]
+RUN_IDS = [c[0] + ("_time_dependent_bc" if "time dependent = true" in c[1] else "")
+           for c in RUN_CASES]
 
 
-@pytest.mark.parametrize("app,text,steps", RUN_CASES, ids=[c[0] for c in RUN_CASES])
+@pytest.mark.parametrize("app,text,steps", RUN_CASES, ids=RUN_IDS)
 def test_run_matches_oracle(app, text, steps):
     """Whole increment loop (ICs, blocks in order, postprocess, rotation)."""
     ctx, orc = Context(app, text), Oracle(app, text)

@@ -482,3 +482,26 @@ def test_newton_bicgstab_matches_dense_newton_and_scipy(oracle_libs):
     # and a BiCGStab step count at a tight tolerance moves by a few with the last bits of
     # its dot products: 5 % slack on top.
     assert abs(ns["linear_iterations"] - scipy_steps) <= its + scipy_steps // 20, (ns, scipy_steps)
+
+
+@pytest.mark.parametrize("time_dependent", [False, True], ids=["fixed", "time_dependent"])
+def test_time_dependent_dirichlet_values(oracle_libs, time_dependent):
+    """Dirichlet values of the test-bed application, g = 0.25 (boundary id + 1) + 0.1 x
+    + 2 t, on the explicit field c and the linear field v after k increments: g at the
+    time of increment k when the conditions are declared time dependent (re-evaluated at
+    the start of every increment, reference problem.cc:428-430), g at t = 0 otherwise
+    (evaluated once).  At a corner the lower boundary id's value holds."""
+    dt, k = 0.01, 4
+    o = oracle("testbed_2d_q2", prm.testbed(dim=2, refinement=3, steps=k, dt=dt,
+                                             dirichlet_faces=(0, 3), time_dependent=time_dependent))
+    assert o.run(k) == 0
+    mesh = make_testbed_mesh(2, 2, 3)
+    x = mesh.node_x()
+    t = k * dt if time_dependent else 0.0
+    lower_x = mesh.coords()[:, 0] == 0                # boundary id 0
+    upper_y = mesh.coords()[:, 1] == mesh.N[1] - 1    # boundary id 3
+    g = np.where(lower_x, 0.25, 1.0) + 0.1 * x[:, 0] + 2.0 * t
+    mask = lower_x | upper_y
+    for f in (0, 1):
+        u = o.get_field(f, -1)
+        assert np.abs(u[mask] - g[mask]).max() <= 1e-13, f
