@@ -1476,55 +1476,69 @@ namespace ppf
               success = state > 0;
               break;
             }
-          double alpha = 1.0, omega = 1.0, rho = 1.0;
+          // Scalars in device slots (see bicgstab_scalar_kernel); the host reads the
+          // residual norms only, and |r.rbar|, |rbar.v| when a breakdown threshold is set.
+          krylov_slots.resize(8);
+          double *ks = krylov_slots.get();
+          hipLaunchKernelGGL(fill_kernel<double>, dim3(1), dim3(64), 0, stream, ks, 1.0,
+                             int64_t(3));
+          const unsigned nb      = grid_for(n, 256);
+          const bool     read_bd = breakdown > 0.0;
+          auto           slot    = [&](int k) {
+            double val = 0.0;
+            PPF_HIP_CHECK(hipMemcpyAsync(&val, ks + k, sizeof(double), hipMemcpyDeviceToHost,
+                                         stream));
+            sync();
+            return val;
+          };
           copy(rbar, r, n);
           bool startup = true, restart = false;
           do
             {
               ++step;
-              double rhobar = dot(r, rbar, n);
-              if (std::fabs(rhobar) < breakdown)
+              dot_to(r, rbar, n, ks + 3);
+              if (read_bd && std::fabs(slot(3)) < breakdown)
                 {
                   restart = true;
                   break;
                 }
-              const double beta = rhobar * alpha / (rho * omega);
-              rho               = rhobar;
+              hipLaunchKernelGGL(bicgstab_scalar_kernel, dim3(1), dim3(1), 0, stream, ks, 0);
               if (startup)
                 {
                   copy(p, r, n);
                   startup = false;
                 }
               else
-                {
-                  axpby(p, 1.0, r, beta, n);          // p = beta p + r
-                  axpby(p, -beta * omega, v, 1.0, n); // p -= beta omega v
-                }
+                hipLaunchKernelGGL(bicgstab_p_kernel, dim3(nb), dim3(256), 0, stream, p, r, v,
+                                   ks, n);
               precondition(b, y, p);
               vmult_fine(b, v, y);
-              rhobar = dot(rbar, v, n);
-              if (std::fabs(rhobar) < breakdown)
+              dot_to(rbar, v, n, ks + 4);
+              if (read_bd && std::fabs(slot(4)) < breakdown)
                 {
                   restart = true;
                   break;
                 }
-              alpha = rho / rhobar;
-              axpby(r, -alpha, v, 1.0, n);
-              res = std::sqrt(dot(r, r, n));
+              hipLaunchKernelGGL(bicgstab_scalar_kernel, dim3(1), dim3(1), 0, stream, ks, 1);
+              hipLaunchKernelGGL(axpy_slot_norm_kernel, dim3(reduce_blocks), dim3(reduce_threads),
+                                 0, stream, reduce_buf.get(), r, v, ks, 1, -1.0, n);
+              res = std::sqrt(finish_reduction(reduce_blocks));
               if (check(res) > 0)
                 {
-                  axpby(x, alpha, y, 1.0, n);
+                  hipLaunchKernelGGL(axpy_slot_kernel, dim3(nb), dim3(256), 0, stream, x, y, ks,
+                                     1, 1.0, n);
                   state = 1;
                   break;
                 }
               precondition(b, z, r);
               vmult_fine(b, t, z);
-              rhobar             = dot(t, r, n);
-              const double t_sq  = dot(t, t, n);
-              omega              = t_sq < std::numeric_limits<double>::min() ? 0.0 : rhobar / t_sq;
-              axpby(x, alpha, y, 1.0, n);
-              axpby(x, omega, z, 1.0, n);
-              axpby(r, -omega, t, 1.0, n);
+              dot_to(t, r, n, ks + 5);
+              dot_to(t, t, n, ks + 6);
+              hipLaunchKernelGGL(bicgstab_scalar_kernel, dim3(1), dim3(1), 0, stream, ks, 2);
+              hipLaunchKernelGGL(bicgstab_x_kernel, dim3(nb), dim3(256), 0, stream, x, y, z, ks,
+                                 n);
+              hipLaunchKernelGGL(axpy_slot_kernel, dim3(nb), dim3(256), 0, stream, r, t, ks, 2,
+                                 -1.0, n);
               if (lp.bicgstab_exact_residual)
                 {
                   vmult_fine(b, t, x);
@@ -1536,6 +1550,7 @@ namespace ppf
               state = check(res);
             }
           while (state == 0);
+          PPF_HIP_CHECK(hipGetLastError());
           if (!restart)
             {
               success = state > 0;
@@ -1565,29 +1580,33 @@ namespace ppf
       unsigned int step   = 0;
       if (!(res <= tol))
         {
+          // The step scalars stay on the device: g.h alternates between slots 0 and 1,
+          // d.Ad is slot 2; alpha and beta are formed by the kernels that use them.  The
+          // host reads only the residual norm, once per step.
+          krylov_slots.resize(8);
+          double *ks = krylov_slots.get();
+          int     cur = 0;
           precondition(b, ls.h.get(), ls.g.get());
           axpby(ls.d.get(), -1.0, ls.h.get(), 0.0, n);
-          double gh = dot(ls.g.get(), ls.h.get(), n);
+          dot_to(ls.g.get(), ls.h.get(), n, ks + cur);
           while (true)
             {
               ++step;
               vmult_fine(b, ls.ad.get(), ls.d.get());
-              const double dad   = dot(ls.d.get(), ls.ad.get(), n);
-              const double alpha = gh / dad;
-              hipLaunchKernelGGL(cg_update_kernel<double>, dim3(reduce_blocks),
+              dot_to(ls.d.get(), ls.ad.get(), n, ks + 2);
+              hipLaunchKernelGGL(cg_update_slots_kernel, dim3(reduce_blocks),
                                  dim3(reduce_threads), 0, stream, reduce_buf.get(), x,
-                                 ls.g.get(), ls.d.get(), ls.ad.get(), alpha, n);
+                                 ls.g.get(), ls.d.get(), ls.ad.get(), ks, cur, 2, n);
               res = std::sqrt(finish_reduction(reduce_blocks));
               if (!std::isfinite(res))
                 throw Error(status_nan, "linear solver produced a non-finite residual");
               if (res <= tol || step >= max_steps)
                 break;
               precondition(b, ls.h.get(), ls.g.get());
-              const double gh_new = dot(ls.g.get(), ls.h.get(), n);
-              const double beta   = gh_new / gh;
-              gh                  = gh_new;
-              hipLaunchKernelGGL(cg_direction_kernel<double>, dim3(grid_for(n, 256)), dim3(256),
-                                 0, stream, ls.d.get(), ls.h.get(), beta, n);
+              dot_to(ls.g.get(), ls.h.get(), n, ks + (1 - cur));
+              hipLaunchKernelGGL(cg_direction_slots_kernel, dim3(grid_for(n, 256)), dim3(256),
+                                 0, stream, ls.d.get(), ls.h.get(), ks, 1 - cur, cur, n);
+              cur = 1 - cur;
             }
         }
       st.iterations     = int(step);
@@ -2497,6 +2516,7 @@ namespace ppf
 
     std::vector<std::vector<DeviceBuffer<double>>> storage;
     std::vector<DeviceBuffer<double>>              dirichlet_values;
+    DeviceBuffer<double>                           krylov_slots; // CG / BiCGStab scalars
     /// Boundary nodes (and the boundary id that set each) of every field's Dirichlet
     /// faces, for re-evaluating time-dependent values.
     std::vector<std::vector<int64_t>>              dirichlet_nodes;

@@ -275,6 +275,107 @@ namespace ppf
       p[i] = -h[i] + beta * p[i];
   }
 
+  /// cg_update_kernel with alpha = s[num] / s[den] read from device slots (the same
+  /// double division the host would do).
+  __global__ void __launch_bounds__(reduce_threads)
+  cg_update_slots_kernel(double *__restrict__ partial, double *__restrict__ x,
+                         double *__restrict__ r, const double *__restrict__ p,
+                         const double *__restrict__ ap, const double *__restrict__ s, int num,
+                         int den, int64_t n)
+  {
+    const double alpha = s[num] / s[den];
+    double       acc   = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+      {
+        x[i] += alpha * p[i];
+        const double ri = r[i] + alpha * ap[i];
+        r[i]            = ri;
+        acc += ri * ri;
+      }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0)
+      partial[blockIdx.x] = acc;
+  }
+
+  /// BiCGStab scalars in device slots: 0 rho, 1 alpha, 2 omega, 3 r.rbar, 4 rbar.v,
+  /// 5 t.r, 6 t.t, 7 beta.  One thread; `op` 0: beta = (r.rbar) alpha / (rho omega),
+  /// rho = r.rbar; 1: alpha = rho / (rbar.v); 2: omega = (t.r) / (t.t), 0 when t.t is
+  /// below the smallest normal double.
+  __global__ void
+  bicgstab_scalar_kernel(double *s, int op)
+  {
+    if (op == 0)
+      {
+        s[7] = s[3] * s[1] / (s[0] * s[2]);
+        s[0] = s[3];
+      }
+    else if (op == 1)
+      s[1] = s[0] / s[4];
+    else
+      s[2] = s[6] < 2.2250738585072014e-308 ? 0.0 : s[5] / s[6];
+  }
+
+  /// p = beta p + r - beta omega v (beta = s[7], omega = s[2]).
+  __global__ void
+  bicgstab_p_kernel(double *__restrict__ p, const double *__restrict__ r,
+                    const double *__restrict__ v, const double *__restrict__ s, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      p[i] = (s[7] * p[i] + r[i]) - s[7] * s[2] * v[i];
+  }
+
+  /// y += sign s[k] x.
+  __global__ void
+  axpy_slot_kernel(double *__restrict__ y, const double *__restrict__ x,
+                   const double *__restrict__ s, int k, double sign, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      y[i] += (sign * s[k]) * x[i];
+  }
+
+  /// y += sign s[k] x, with per-block partial sums of y.y (grid-stride).
+  __global__ void __launch_bounds__(reduce_threads)
+  axpy_slot_norm_kernel(double *__restrict__ partial, double *__restrict__ y,
+                        const double *__restrict__ x, const double *__restrict__ s, int k,
+                        double sign, int64_t n)
+  {
+    const double a   = sign * s[k];
+    double       acc = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+      {
+        const double yi = y[i] + a * x[i];
+        y[i]            = yi;
+        acc += yi * yi;
+      }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0)
+      partial[blockIdx.x] = acc;
+  }
+
+  /// x += alpha y + omega z (alpha = s[1], omega = s[2]).
+  __global__ void
+  bicgstab_x_kernel(double *__restrict__ x, const double *__restrict__ y,
+                    const double *__restrict__ z, const double *__restrict__ s, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      x[i] += s[1] * y[i] + s[2] * z[i];
+  }
+
+  /// cg_direction_kernel with beta = s[num] / s[den] read from device slots.
+  __global__ void
+  cg_direction_slots_kernel(double *__restrict__ p, const double *__restrict__ h,
+                            const double *__restrict__ s, int num, int den, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      p[i] = -h[i] + (s[num] / s[den]) * p[i];
+  }
+
   /// Chebyshev step (Varga / Saad 12.1 recurrence as used by deal.II's
   /// PreconditionChebyshev): d = c1 * d + c2 * P^{-1} (b - A x);  x += d.
   /// With `first` set, d = c2 * P^{-1}(b - A x).
