@@ -23,7 +23,9 @@ Printed JSON line:
                FP32-forward) with their algorithmic GB/s (16 and 40 B per DoF and step);
                implicit Allen-Cahn 2D (Newton blocks): time, Newton updates and CG steps
                per increment, without and with GMG; implicit Cahn-Hilliard 2D (Newton
-               with BiCGStab) at the reference application's parameters.
+               with BiCGStab) at the reference application's parameters; coupled
+               precipitate 3D: increment time in FP64 and FP32-operator/FP64-state, and
+               the difference between the two.
 """
 
 from __future__ import annotations
@@ -273,6 +275,47 @@ def adjoint_extra(torch):
     return res
 
 
+def precipitate_extra(torch):
+    """Coupled Allen-Cahn/Cahn-Hilliard precipitate configuration (Q1, 192^3 cells on
+    [0,192]^3, periodic, fields c, mu, eta): increment time in FP64 and with FP32 operator
+    arithmetic on the FP64 state, and the largest difference between the two states after
+    the same increments."""
+    from phasefield_amd import Context, prm
+    from phasefield_amd.capi import AGE_LATEST
+
+    n_warm, n_inc = 5, 40
+    res = {"workload": "Coupled Allen-Cahn/Cahn-Hilliard precipitate 3D, Q1, 192^3 cells on "
+                       f"[0,192]^3, fields c, mu, eta, dt=5e-3, {n_inc} increments after "
+                       f"{n_warm} warm-up increments",
+           "bytes_per_dof_per_increment": 72}
+    states = {}
+    for label, precision in (("fp64", "double"), ("fp32_operator_fp64_state", "single")):
+        ctx = Context("precipitate_3d_q1",
+                      prm.config_prm("precipitate_3d", steps=n_warm + n_inc + 1, outputs=0))
+        ctx.set_option("explicit_precision", precision)
+        if ctx.run(n_warm) != 0:
+            raise RuntimeError("precipitate warm-up stopped")
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if ctx.run(n_warm + n_inc) != 0:
+            raise RuntimeError("precipitate increments stopped")
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        n_d = ctx.info.n_dofs
+        res[label] = {
+            "us_per_increment": dt / n_inc * 1e6,
+            "dofs_per_s": 3 * n_d * n_inc / dt,
+            "algorithmic_gbs": 72 * n_d * n_inc / dt / 1e9,
+        }
+        states[label] = [ctx.get_field(f, AGE_LATEST) for f in range(3)]
+        del ctx
+    res["fp32_operator_max_abs_difference"] = max(
+        float(np.abs(a - b).max()) for a, b in zip(states["fp64"], states["fp32_operator_fp64_state"]))
+    return res
+
+
 def newton_extra(torch):
     """Implicit Allen-Cahn (Newton + CG per increment) at the reference application's
     parameters (applications/allen_cahn/implicit/parameters.prm: 256^2 cells on
@@ -435,6 +478,7 @@ def main():
         torch.cuda.empty_cache()
         extra["adjoint"] = adjoint_extra(torch)
         extra["newton"] = newton_extra(torch)
+        extra["precipitate"] = precipitate_extra(torch)
     cpu = cpu_baseline(args.cpu_seconds) if args.cpu_seconds > 0 else None
 
     line = {
