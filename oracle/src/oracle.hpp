@@ -1133,9 +1133,27 @@ namespace oracle
             apply(b, true, LL.mesh, LL.cons, LL.jxw, LL.fields, x.data(), y.data(), scaling_of(b),
                   false);
           };
-          estimate(A, mg[i].sm, start_vector(L.mesh, int(bf.size())), lp.chebyshev_parameters);
+          // The smoothers are initialized once (reference linear_solver.h:135-147) and
+          // deal.II's PreconditionChebyshev estimates its eigenvalues at its first
+          // application only: the estimates of the first solve are kept.
+          auto &kept = mg_eigenvalues[b];
+          if (kept.size() == std::size_t(depth))
+            {
+              mg[i].sm.lmax = kept[i].first;
+              mg[i].sm.lmin = kept[i].second;
+            }
+          else
+            estimate(A, mg[i].sm, start_vector(L.mesh, int(bf.size())), lp.chebyshev_parameters);
+        }
+      auto &kept = mg_eigenvalues[b];
+      if (kept.size() != std::size_t(depth))
+        {
+          kept.clear();
+          for (int i = 0; i < depth; ++i)
+            kept.emplace_back(mg[i].sm.lmax, mg[i].sm.lmin);
         }
     }
+    std::map<int, std::vector<std::pair<double, double>>> mg_eigenvalues; // per block
 
     void
     v_cycle(int b, int li, std::vector<double> &u, const std::vector<double> &defect) const
@@ -1202,7 +1220,8 @@ namespace oracle
       chebyshev(A, mg[li].sm, deg, u, defect, false);
     }
 
-    Smoother fine_cheb;
+    Smoother                                     fine_cheb;
+    std::map<int, std::pair<double, double>>     cheb_eigenvalues; // per block
     std::map<int, std::vector<double>> homog; // persistent initial guesses per block
     std::map<int, Stats>               last_stats;
 
@@ -1331,8 +1350,20 @@ namespace oracle
       };
       if (lp.preconditioner == prismspf::Chebyshev)
         {
+          // Diagonal re-evaluated every solve (reference linear_solver.h:362-367);
+          // eigenvalues estimated at the preconditioner's first application only.
           fine_cheb.inv_diag = inverse_diagonal(b, top, cons, jxw, storage, sc);
-          estimate(A, fine_cheb, start_vector(top, int(bf.size())), lp.chebyshev_parameters);
+          auto it = cheb_eigenvalues.find(b);
+          if (it == cheb_eigenvalues.end())
+            {
+              estimate(A, fine_cheb, start_vector(top, int(bf.size())), lp.chebyshev_parameters);
+              cheb_eigenvalues[b] = {fine_cheb.lmax, fine_cheb.lmin};
+            }
+          else
+            {
+              fine_cheb.lmax = it->second.first;
+              fine_cheb.lmin = it->second.second;
+            }
         }
       else if (lp.preconditioner == prismspf::GMG)
         setup_mg(b);

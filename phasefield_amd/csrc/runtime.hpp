@@ -1201,6 +1201,7 @@ namespace ppf
       DeviceBuffer<double>                inv_diag_d;
       DeviceBuffer<float>                 inv_diag_f;
       double                              lambda_max = 0.0, lambda_min = 0.0;
+      bool                                eigenvalues_estimated = false;
       std::vector<DeviceBuffer<double>>   field_d[PPF_MAX_FIELDS];
       std::vector<DeviceBuffer<float>>    field_f[PPF_MAX_FIELDS];
       DeviceBuffer<double>                tmp_d[2]; // separable transfers to the coarser level
@@ -1214,6 +1215,7 @@ namespace ppf
       DeviceBuffer<double> x_homog, inhom, rhs, g, h, d, ad, tmp;
       DeviceBuffer<double> inv_diag; // Chebyshev preconditioner on the fine level
       double               cheb_lmax = 0.0, cheb_lmin = 0.0;
+      bool                 cheb_eigenvalues_estimated = false;
       DeviceBuffer<double> cheb_d, cheb_ax;
       DeviceBuffer<double> bs_r, bs_rbar, bs_p, bs_y, bs_z, bs_t, bs_v; // BiCGStab
       std::vector<Level>   levels; // coarsest first
@@ -1623,15 +1625,20 @@ namespace ppf
       const auto   kind = lin_params(b).preconditioner;
       if (kind == prismspf::Chebyshev)
         {
-          // The reference re-evaluates the diagonal every solve (linear_solver.h:362-378).
+          // The reference re-evaluates the diagonal every solve (linear_solver.h:362-378)
+          // into the matrix its PreconditionChebyshev was initialized with once
+          // (initialize_chebyshev, :446-458).  deal.II estimates the eigenvalues at that
+          // preconditioner's first application and keeps them: so does this path.
           ls.inv_diag.resize(ls.n);
           ls.cheb_d.resize(ls.n);
           ls.cheb_ax.resize(ls.n);
           inverse_diagonal<double>(b, fine_fields(), linear_scaling(b), ls.inv_diag.get());
-          estimate_eigenvalues<double>(
-            [&](double *dst, const double *src) { vmult_fine(b, dst, src); }, ls.inv_diag.get(),
-            ls.n, lin_params(b).chebyshev_parameters, ls.cheb_lmax, ls.cheb_lmin, ls.tmp.get(),
-            ls.cheb_d.get(), ls.cheb_ax.get(), ls.h.get());
+          if (!ls.cheb_eigenvalues_estimated)
+            estimate_eigenvalues<double>(
+              [&](double *dst, const double *src) { vmult_fine(b, dst, src); },
+              ls.inv_diag.get(), ls.n, lin_params(b).chebyshev_parameters, ls.cheb_lmax,
+              ls.cheb_lmin, ls.tmp.get(), ls.cheb_d.get(), ls.cheb_ax.get(), ls.h.get());
+          ls.cheb_eigenvalues_estimated = true;
         }
       else if (kind == prismspf::GMG)
         setup_multigrid(b);
@@ -1819,7 +1826,12 @@ namespace ppf
 
     /// MGContext::init (reference linear_solver.h:86-165): level operators, level
     /// diagonals, Chebyshev smoothers, transfers.  Levels span
-    /// [global_refinement - mg_depth + 1, global_refinement].
+    /// [global_refinement - mg_depth + 1, global_refinement].  Called before every
+    /// solve: the level fields and diagonals are refreshed each time (as
+    /// do_linear_solve re-evaluates them, linear_solver.h:369-379); the smoothers'
+    /// eigenvalue estimates are made once per level, at the first solve, because the
+    /// reference initializes its smoothers once (:135-147) and deal.II's
+    /// PreconditionChebyshev estimates at its first application only.
     void
     setup_multigrid(int b)
     {
@@ -1923,9 +1935,12 @@ namespace ppf
       const auto   &lp = lin_params(b);
       const int64_t n  = int64_t(blocks[b].field_indices.size()) * l.grid.n_nodes;
       inverse_diagonal<T>(b, level_fields<T>(l), level_scaling(b), linv<T>(l));
+      if (l.eigenvalues_estimated)
+        return;
       estimate_eigenvalues<T>([&](T *dst, const T *src) { level_vmult<T>(b, l, dst, src); },
                               linv<T>(l), n, lp.chebyshev_parameters, l.lambda_max, l.lambda_min,
                               lvec<T>(l, 2), lvec<T>(l, 3), lvec<T>(l, 4), lvec<T>(l, 5));
+      l.eigenvalues_estimated = true;
     }
 
     /// PreconditionMG::vmult: one V-cycle on the residual g.
