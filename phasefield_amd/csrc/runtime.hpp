@@ -1743,14 +1743,16 @@ namespace ppf
       const double delta = 0.5 * (lmax - lmin);
       const double sigma = theta / delta;
       double       rho   = 1.0 / sigma;
-      if (zero_start)
-        fill(x, T(0), n);
       for (unsigned int k = 0; k < deg; ++k)
         {
           if (k == 0 && zero_start)
-            fill(ax, T(0), n);
-          else
-            A(ax, x);
+            {
+              // From x = 0 the residual is b: no operator application, no clearing.
+              hipLaunchKernelGGL(chebyshev_first_zero_kernel<T>, dim3(grid_for(n, 256)),
+                                 dim3(256), 0, stream, x, d, rhs, inv_diag, T(1.0 / theta), n);
+              continue;
+            }
+          A(ax, x);
           double c1 = 0.0, c2;
           if (k == 0)
             c2 = 1.0 / theta;

@@ -376,6 +376,21 @@ namespace ppf
       p[i] = -h[i] + (s[num] / s[den]) * p[i];
   }
 
+  /// First Chebyshev step from x = 0 (A x = 0): d = c2 P^{-1} b, x = d.  Same values as
+  /// chebyshev_step_kernel with x = ax = 0, without reading or clearing them.
+  template <typename T>
+  __global__ void
+  chebyshev_first_zero_kernel(T *__restrict__ x, T *__restrict__ d, const T *__restrict__ b,
+                              const T *__restrict__ inv_diag, T c2, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+      return;
+    const T di = c2 * inv_diag[i] * b[i];
+    d[i]       = di;
+    x[i]       = di;
+  }
+
   /// Chebyshev step (Varga / Saad 12.1 recurrence as used by deal.II's
   /// PreconditionChebyshev): d = c1 * d + c2 * P^{-1} (b - A x);  x += d.
   /// With `first` set, d = c2 * P^{-1}(b - A x).
