@@ -50,7 +50,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="target duration of the CPU-baseline sample (0 disables it)")
-    p.add_argument("--no-extra", action="store_true", help="skip the Poisson measurements")
+    p.add_argument("--no-extra", action="store_true", help="skip every extra measurement")
+    p.add_argument("--extras", default="poisson,adjoint,newton,precipitate",
+                   help="comma-separated extra measurements to run")
     p.add_argument("--kernel-reps", type=int, default=50)
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run a plain device copy of 16,974,593 doubles (counter "
@@ -474,11 +476,14 @@ def main():
         del src, dst
     extra = None
     if not args.no_extra:
-        extra = poisson_extra(torch)
+        wanted = set(args.extras.split(","))
+        extra = poisson_extra(torch) if "poisson" in wanted else {}
         torch.cuda.empty_cache()
-        extra["adjoint"] = adjoint_extra(torch)
-        extra["newton"] = newton_extra(torch)
-        extra["precipitate"] = precipitate_extra(torch)
+        for name, fn in (("adjoint", adjoint_extra), ("newton", newton_extra),
+                         ("precipitate", precipitate_extra)):
+            if name in wanted:
+                extra[name] = fn(torch)
+                torch.cuda.empty_cache()
     cpu = cpu_baseline(args.cpu_seconds) if args.cpu_seconds > 0 else None
 
     line = {
