@@ -5,7 +5,7 @@ BiCGStab is deal.II's SolverBicgstab as the reference selects it (`solver type =
 bicgstab`, reference include/prismspf/solvers/linear_solver.h:461-490), restated on
 both sides.  Per increment: the same number of Newton updates, BiCGStab steps within
 15 % (see CASES), the new (c, mu) and the post-processed free-energy density within 1e-9
-relative at tight solver tolerances.  Cases: exact and recurrence residuals, no preconditioner and Chebyshev,
+relative at tight solver tolerances.  Cases: exact and recurrence residuals, no preconditioner, Chebyshev and GMG,
 natural and periodic boundaries, the reference's own parameters (dt = 5e-4, tolerances
 1e-6) on a 129 x 129-node lattice whose operators run on the tiled stencil kernel, and
 a solver type this path does not implement (an error, not a fallback).
@@ -43,6 +43,13 @@ CASES = {
                                                     newton_tolerance=1e-10,
                                                     preconditioner="Chebyshev"), 1e-9),
     "reference_parameters_129": (prm.cahn_hilliard_implicit(refinement=6, steps=3), 5e-4),
+    # Two-field multigrid levels (level operators of a non-symmetric Jacobian, the
+    # current (c, mu) injected into the levels before every linear solve).
+    "gmg_two_fields": (prm.cahn_hilliard_implicit(refinement=5, steps=3, dt=1e-3,
+                                                  linear_tolerance=1e-12, newton_tolerance=1e-10,
+                                                  preconditioner="GMG").replace(
+        "  set preconditioner type = GMG\n", "  set preconditioner type = GMG\n  set mg depth = 3\n"),
+        1e-9),
 }
 
 
