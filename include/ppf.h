@@ -103,7 +103,9 @@ int ppf_field_name(ppf_ctx *ctx, int field, char *buf, int len);
  * shape; "stencil_store_policy" = auto|temporal|nontemporal sets the cache policy of
  * its output stores (auto: non-temporal when the launch's arrays exceed 192 MiB);
  * "synchronize" = 1|0 (0 leaves ppf_apply_operator asynchronous on the
- * context's stream);
+ * context's stream); "write_output" = 0|1 (1: at every output increment ppf_run and
+ * ppf_solve_increment write the solution file of the `output` parameters,
+ * <directory>/<file name>_<increment>.vtu, as SolutionOutput does, problem.cc:500-515);
  * "mg_level_precision" = double|single, "mg_coarse_solver" = smoother|cg,
  * "linear_tolerance" = <value> and "linear_max_iterations" = <n> (applied to every
  * linear block, overriding the parameter file); "linear_initial_guess" =

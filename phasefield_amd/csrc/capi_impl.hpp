@@ -286,6 +286,12 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
           throw ppf::Error(ppf::status_invalid, "explicit_precision must be double|single");
         c.rt.explicit_fp32 = v == "single";
       }
+    else if (k == "write_output")
+      {
+        if (v != "0" && v != "1")
+          throw ppf::Error(ppf::status_invalid, "write_output must be 0|1");
+        c.rt.write_output = v == "1";
+      }
     else if (k == "linear_initial_guess")
       {
         if (v != "previous" && v != "zero")

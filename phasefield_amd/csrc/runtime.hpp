@@ -16,6 +16,7 @@
 #include "operator_plan.hpp"
 #include "stencil_kernels.hpp"
 #include "vector_kernels.hpp"
+#include "vtu_writer.hpp"
 
 #include <prismspf/core/field_attributes.h>
 #include <prismspf/core/phase_field_tools.h>
@@ -28,7 +29,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <filesystem>
 #include <functional>
+#include <iomanip>
 #include <limits>
 #include <map>
 #include <memory>
@@ -446,6 +449,47 @@ namespace ppf
               const int64_t i0 = j0 % g.nodes[0], i1 = j1 % g.nodes[1], i2 = j2 % g.nodes[2];
               host[j0 + f0 * (j1 + f1 * j2)] = uniq[node_index(g, int(i0), int(i1), int(i2))];
             }
+    }
+
+    /// SolutionOutput at an output increment (reference problem.cc:500-515 and
+    /// solution_output.h:53-126, VTU branch): every field's current state, to
+    /// `<directory>/<file name>_<increment>.vtu`, the increment zero-padded to the digits
+    /// of the number of increments.  Written only when the option write_output is set.
+    void
+    write_output_files()
+    {
+      const auto &op = inputs.output_parameters;
+      if (op.file_type != "vtu")
+        throw Error(status_unsupported, "output file type '" + op.file_type +
+                                          "' is not supported (vtu)");
+      if (op.subdivisions != 0 && op.subdivisions != unsigned(degree))
+        throw Error(status_unsupported,
+                    "output subdivisions other than the element degree are not supported");
+      const unsigned int n_inc  = inputs.temporal_discretization.n_increments;
+      const int          digits = n_inc == 0 ? 1 : int(std::floor(std::log10(double(n_inc)))) + 1;
+      std::ostringstream name;
+      name << op.file_name << "_" << std::setw(digits) << std::setfill('0')
+           << timer.get_increment() << ".vtu";
+      const std::filesystem::path path = std::filesystem::path(op.directory) / name.str();
+      std::filesystem::create_directories(path.parent_path().empty() ? "." : path.parent_path());
+      int64_t n[3] = {1, 1, 1};
+      double  lower[3] = {0, 0, 0}, step[3] = {0, 0, 0};
+      for (int d = 0; d < dim; ++d)
+        {
+          n[d]     = int64_t(degree) * fine.cells[d] + 1;
+          lower[d] = fine.lower[d];
+          step[d]  = fine.h[d] / degree;
+        }
+      std::vector<std::string>         names;
+      std::vector<std::vector<double>> values;
+      for (std::size_t f = 0; f < fields.size(); ++f)
+        {
+          names.push_back(fields[f].name);
+          values.emplace_back(full_size(fine));
+          get_field(int(f), 0, values.back().data(), int64_t(values.back().size()));
+        }
+      write_vtu(path.string(), dim, n, lower, step, names, values, timer.get_time(),
+                timer.get_increment());
     }
 
     // ---------------------------------------------------------------- utilities
@@ -2357,7 +2401,11 @@ namespace ppf
       if (status == 0 && pde_d->get_user_stop())
         status = 3;
       if (do_output || status != 0)
-        log_norms();
+        {
+          if (write_output)
+            write_output_files();
+          log_norms();
+        }
       rotate_all();
       return status;
     }
@@ -2416,6 +2464,8 @@ namespace ppf
         unrotate_all();
         timer   = before.timer;
         rotated = before.rotated;
+        if (write_output)
+          write_output_files();
         log_norms();
         rotate_all();
         timer.increment();
@@ -2534,6 +2584,7 @@ namespace ppf
     std::vector<std::vector<DeviceBuffer<double>>> storage;
     std::vector<DeviceBuffer<double>>              dirichlet_values;
     DeviceBuffer<double>                           krylov_slots; // CG / BiCGStab scalars
+    bool write_output = false; // option write_output: solution files at output increments
     /// Boundary nodes (and the boundary id that set each) of every field's Dirichlet
     /// faces, for re-evaluating time-dependent values.
     std::vector<std::vector<int64_t>>              dirichlet_nodes;

@@ -200,6 +200,13 @@ struct TemporalDiscretization
 struct OutputParameters
 {
   std::set<unsigned int> output_list;
+  /// File output (reference src/user_inputs/io_parameters.cc:10-110): `file type`
+  /// vtu|vtk|pvtu|xdmf, `subdivisions` (0: the element degree), `directory` (alias
+  /// `folder name`), `file name` (the prefix before the increment).
+  std::string  file_type    = "vtu";
+  unsigned int subdivisions = 0;
+  std::string  directory    = "solutions";
+  std::string  file_name    = "solution";
 
   [[nodiscard]] bool
   should_output(unsigned int increment) const
@@ -379,6 +386,14 @@ private:
         }
       for (const auto &s : prm::split(f.get("output/", {"list"}, ""), ','))
         list.insert(std::stoul(s));
+      output_parameters.file_type = f.get("output/", {"file type"}, "vtu");
+      if (output_parameters.file_type != "vtu" && output_parameters.file_type != "vtk" &&
+          output_parameters.file_type != "pvtu" && output_parameters.file_type != "xdmf")
+        throw std::invalid_argument("Invalid output file type: " + output_parameters.file_type);
+      output_parameters.subdivisions = std::stoul(f.get("output/", {"subdivisions"}, "0"));
+      output_parameters.directory =
+        f.get("output/", {"directory", "folder name"}, "solutions");
+      output_parameters.file_name = f.get("output/", {"file name"}, "solution");
     }
 
     misc_parameters.random_seed =

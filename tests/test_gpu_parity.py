@@ -373,3 +373,23 @@ def test_explicit_matches_adjoint_forward(device):
     d0, d4 = dev(phi0), torch.zeros(ctx.info.n_nodes, dtype=torch.float64, device="cuda")
     ctx.adjoint_forward(d0.data_ptr(), 1.0, 1.5, 4, FP64, d4.data_ptr())
     assert rel_err(d4.cpu().numpy(), phi4) <= 1e-12
+
+
+def test_solution_files_at_output_increments(device, tmp_path):
+    """Option write_output: the reference's solution files (`<directory>/<file
+    name>_<increment>.vtu` at every output increment, problem.cc:500-515) hold every
+    field's state of that increment."""
+    from tests.test_vtu_writer import read_vtu
+
+    text = prm.allen_cahn(dim=2, size=16.0, refinement=4, steps=4, dt=0.01, outputs=2)
+    text = text.replace("subsection output\n", f"subsection output\n  set directory = {tmp_path}\n")
+    ctx = Context("allen_cahn_2d_q1", text)
+    ctx.set_option("write_output", "1")
+    assert ctx.run(4) == 0
+    names = sorted(p.name for p in tmp_path.glob("*.vtu"))
+    assert names == ["solution_0.vtu", "solution_2.vtu", "solution_4.vtu"]
+    arrs, n_points, _ = read_vtu(tmp_path / "solution_4.vtu")
+    assert n_points == ctx.info.n_dofs
+    for f, name in enumerate(("n", "f_tot")):
+        np.testing.assert_array_equal(arrs[name], ctx.get_field(f, AGE_LATEST).ravel())
+    assert arrs["CYCLE"][0] == 4
