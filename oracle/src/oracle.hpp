@@ -1334,6 +1334,165 @@ namespace oracle
       return st;
     }
 
+    /// deal.II SolverRichardson (lac/solver_richardson.h), restated: r = b - A x,
+    /// d = P^{-1} r, check |r| (or |d|) at the step count, x += omega d.
+    template <class AOp, class POp>
+    static Stats
+    richardson(AOp &A, POp &P, std::vector<double> &x, const std::vector<double> &b, double tol,
+               unsigned max_steps, double omega, bool preconditioned_residual)
+    {
+      const std::size_t   n = x.size();
+      std::vector<double> r(n), d(n);
+      Stats               st;
+      st.tol        = tol;
+      unsigned step = 0;
+      for (;; ++step)
+        {
+          A(x, r);
+          for (std::size_t i = 0; i < n; ++i)
+            r[i] = b[i] - r[i];
+          P(r, d);
+          const double res = std::sqrt(preconditioned_residual ? dot(d, d) : dot(r, r));
+          if (step == 0)
+            st.r0 = res;
+          st.r = res;
+          if (res <= tol || step >= max_steps)
+            {
+              st.converged = res <= tol;
+              break;
+            }
+          for (std::size_t i = 0; i < n; ++i)
+            x[i] += omega * d[i];
+        }
+      st.iterations = int(step);
+      return st;
+    }
+
+    /// deal.II SolverGMRES (lac/solver_gmres.h), restated: restarted GMRES, basis of
+    /// `max basis size` vectors, left (default) or right preconditioning, modified
+    /// Gram-Schmidt, Givens rotations; each cycle checks the recomputed (preconditioned,
+    /// for left preconditioning) residual at the accumulated step count and each Arnoldi
+    /// step the least-squares estimate.
+    template <class AOp, class POp>
+    static Stats
+    gmres(AOp &A, POp &P, std::vector<double> &x, const std::vector<double> &b, double tol,
+          unsigned max_steps, int m, bool right)
+    {
+      const std::size_t                n = x.size();
+      std::vector<std::vector<double>> V(m + 1, std::vector<double>(n));
+      std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
+      std::vector<double>              w(n), t(n), cs(m), sn(m), g(m + 1), y(m);
+      Stats                            st;
+      st.tol        = tol;
+      unsigned its  = 0;
+      double   rho  = 0.0;
+      int      state = 0;
+      bool     first = true;
+      auto     check = [&](double v) { return v <= tol ? 1 : (its >= max_steps ? -1 : 0); };
+      while (true)
+        {
+          if (!right)
+            {
+              A(x, t);
+              for (std::size_t i = 0; i < n; ++i)
+                t[i] = b[i] - t[i];
+              P(t, V[0]);
+            }
+          else
+            {
+              A(x, V[0]);
+              for (std::size_t i = 0; i < n; ++i)
+                V[0][i] = b[i] - V[0][i];
+            }
+          rho = std::sqrt(dot(V[0], V[0]));
+          if (first)
+            st.r0 = rho;
+          first = false;
+          state = check(rho);
+          if (state != 0)
+            break;
+          for (auto &v : V[0])
+            v *= 1.0 / rho;
+          std::fill(g.begin(), g.end(), 0.0);
+          g[0]  = rho;
+          int j = 0;
+          while (j < m)
+            {
+              ++its;
+              if (!right)
+                {
+                  A(V[j], t);
+                  P(t, w);
+                }
+              else
+                {
+                  P(V[j], t);
+                  A(t, w);
+                }
+              for (int i = 0; i <= j; ++i)
+                {
+                  const double h = dot(w, V[i]);
+                  H[i][j]        = h;
+                  for (std::size_t k = 0; k < n; ++k)
+                    w[k] -= h * V[i][k];
+                }
+              const double hn = std::sqrt(dot(w, w));
+              H[j + 1][j]     = hn;
+              if (hn > 0.0)
+                for (std::size_t k = 0; k < n; ++k)
+                  V[j + 1][k] = w[k] / hn;
+              for (int i = 0; i < j; ++i)
+                {
+                  const double a = H[i][j], c = H[i + 1][j];
+                  H[i][j]        = cs[i] * a + sn[i] * c;
+                  H[i + 1][j]    = -sn[i] * a + cs[i] * c;
+                }
+              const double a = H[j][j], c = H[j + 1][j];
+              const double r = std::sqrt(a * a + c * c);
+              cs[j]          = r > 0.0 ? a / r : 1.0;
+              sn[j]          = r > 0.0 ? c / r : 0.0;
+              H[j][j]        = r;
+              H[j + 1][j]    = 0.0;
+              g[j + 1]       = -sn[j] * g[j];
+              g[j]           = cs[j] * g[j];
+              rho            = std::fabs(g[j + 1]);
+              ++j;
+              state = check(rho);
+              if (state != 0 || hn == 0.0)
+                break;
+            }
+          for (int i = j - 1; i >= 0; --i)
+            {
+              double s = g[i];
+              for (int k = i + 1; k < j; ++k)
+                s -= H[i][k] * y[k];
+              y[i] = s / H[i][i];
+            }
+          if (!right)
+            {
+              for (int i = 0; i < j; ++i)
+                for (std::size_t k = 0; k < n; ++k)
+                  x[k] += y[i] * V[i][k];
+            }
+          else
+            {
+              std::fill(w.begin(), w.end(), 0.0);
+              for (int i = 0; i < j; ++i)
+                for (std::size_t k = 0; k < n; ++k)
+                  w[k] += y[i] * V[i][k];
+              P(w, t);
+              for (std::size_t k = 0; k < n; ++k)
+                x[k] += t[k];
+            }
+          if (state != 0)
+            break;
+        }
+      st.iterations = int(its);
+      st.r          = rho;
+      st.converged  = state > 0;
+      return st;
+    }
+
     /// do_linear_solve (reference linear_solver.h:348-402): the block's preconditioner,
     /// set up for this solve, and deal.II SolverCG's recurrences from the initial guess x
     /// (absolute residual tolerance = tolerance value * normalization).
@@ -1387,9 +1546,15 @@ namespace oracle
       if (lp.solver_type == "bicgstab")
         return bicgstab(A, P, x, rhs, st.tol, lp.max_iterations, lp.bicgstab_exact_residual,
                         lp.bicgstab_breakdown);
+      if (lp.solver_type == "gmres")
+        return gmres(A, P, x, rhs, st.tol, lp.max_iterations, int(lp.gmres_max_basis_size),
+                     lp.gmres_right_preconditioning);
+      if (lp.solver_type == "richardson")
+        return richardson(A, P, x, rhs, st.tol, lp.max_iterations, lp.richardson_omega,
+                          lp.richardson_use_preconditioned_residual);
       if (lp.solver_type != "cg")
         throw std::invalid_argument("linear solver type " + lp.solver_type +
-                                    " is not part of the oracle (cg, bicgstab)");
+                                    " is not part of the oracle (cg, bicgstab, gmres, richardson)");
       std::vector<double> g(n), h(n), d(n), ad(n);
       A(x, g);
       for (int64_t i = 0; i < n; ++i)

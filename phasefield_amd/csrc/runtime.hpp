@@ -1262,6 +1262,8 @@ namespace ppf
       bool                 cheb_eigenvalues_estimated = false;
       DeviceBuffer<double> cheb_d, cheb_ax;
       DeviceBuffer<double> bs_r, bs_rbar, bs_p, bs_y, bs_z, bs_t, bs_v; // BiCGStab
+      std::vector<DeviceBuffer<double>> gm_v;                          // GMRES basis
+      DeviceBuffer<double>              gm_w, gm_t;
       std::vector<Level>   levels; // coarsest first
       SolveStats           stats;
       NewtonStats          newton;
@@ -1474,8 +1476,196 @@ namespace ppf
         return cg(b, x, rhs, tol, max_steps);
       if (type == "bicgstab")
         return bicgstab(b, x, rhs, tol, max_steps);
-      throw Error(status_unsupported,
-                  "linear solver type '" + type + "' is not supported (cg, bicgstab)");
+      if (type == "gmres")
+        return gmres(b, x, rhs, tol, max_steps);
+      if (type == "richardson")
+        return richardson(b, x, rhs, tol, max_steps);
+      throw Error(status_unsupported, "linear solver type '" + type +
+                                        "' is not supported (cg, bicgstab, gmres, richardson)");
+    }
+
+    /// SolverControl::check: 1 success (value <= tol), -1 failure (step >= max_steps or
+    /// not finite), 0 iterate.
+    static int
+    control_check(double value, double tol, unsigned int step, unsigned int max_steps)
+    {
+      if (!std::isfinite(value))
+        throw Error(status_nan, "linear solver produced a non-finite residual");
+      if (value <= tol)
+        return 1;
+      return step >= max_steps ? -1 : 0;
+    }
+
+    /// deal.II SolverRichardson (lac/solver_richardson.h; `solver type = richardson`),
+    /// restated: each step forms r = b - A x and d = P^{-1} r, checks |r| (or |d| with
+    /// `use preconditioned residual`) at the step count so far, then x += omega d.
+    SolveStats
+    richardson(int b, double *x, const double *rhs, double tol, unsigned int max_steps)
+    {
+      LinearState &ls = linear_state(b);
+      const int64_t n = ls.n;
+      const auto   &lp = lin_params(b);
+      SolveStats    st;
+      st.tolerance = tol;
+      double   res  = 0.0;
+      int      state = 0;
+      unsigned step = 0;
+      for (;; ++step)
+        {
+          vmult_fine(b, ls.g.get(), x);
+          axpby(ls.g.get(), 1.0, rhs, -1.0, n); // r = b - A x
+          precondition(b, ls.h.get(), ls.g.get());
+          const double *crit = lp.richardson_use_preconditioned_residual ? ls.h.get() : ls.g.get();
+          res = std::sqrt(dot(crit, crit, n));
+          if (step == 0)
+            st.initial_residual = res;
+          state = control_check(res, tol, step, max_steps);
+          if (state != 0)
+            break;
+          axpby(x, lp.richardson_omega, ls.h.get(), 1.0, n);
+        }
+      st.iterations     = int(step);
+      st.final_residual = res;
+      st.converged      = state > 0 ? 1 : 0;
+      return st;
+    }
+
+    /// deal.II SolverGMRES (lac/solver_gmres.h; `solver type = gmres`), restated:
+    /// restarted GMRES with a basis of `max basis size` vectors, left preconditioning
+    /// (the residual checked is |P^{-1}(b - A x)|) or right preconditioning.  Each cycle
+    /// checks the recomputed residual at the accumulated step count, then each Arnoldi
+    /// step checks the least-squares estimate |g_{j+1}| (`use default residual`).  The
+    /// basis is orthogonalized by modified Gram-Schmidt with the inner products and
+    /// updates on the device; the Hessenberg column (j + 2 values) is read once per step
+    /// and the Givens rotations run on the host.
+    SolveStats
+    gmres(int b, double *x, const double *rhs, double tol, unsigned int max_steps)
+    {
+      LinearState &ls = linear_state(b);
+      const int64_t n = ls.n;
+      const auto   &lp = lin_params(b);
+      if (!lp.gmres_use_default_residual)
+        throw Error(status_unsupported, "GMRES with use default residual = false");
+      const int  m     = int(lp.gmres_max_basis_size);
+      const bool right = lp.gmres_right_preconditioning;
+      if (int(ls.gm_v.size()) != m + 1)
+        ls.gm_v.resize(m + 1);
+      for (auto &v : ls.gm_v)
+        v.resize(n);
+      ls.gm_w.resize(n);
+      ls.gm_t.resize(n);
+      krylov_slots.resize(std::max<int64_t>(8, m + 2));
+      double                          *ks = krylov_slots.get();
+      std::vector<double>              col(m + 2), cs(m), sn(m), g(m + 1), y(m);
+      std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
+      SolveStats                       st;
+      st.tolerance   = tol;
+      unsigned its   = 0;
+      double   rho   = 0.0;
+      int      state = 0;
+      bool     first = true;
+      double  *w = ls.gm_w.get(), *t = ls.gm_t.get();
+      while (true)
+        {
+          double *v0 = ls.gm_v[0].get();
+          if (!right)
+            {
+              vmult_fine(b, t, x);
+              axpby(t, 1.0, rhs, -1.0, n); // b - A x
+              precondition(b, v0, t);
+            }
+          else
+            {
+              vmult_fine(b, v0, x);
+              axpby(v0, 1.0, rhs, -1.0, n);
+            }
+          rho = std::sqrt(dot(v0, v0, n));
+          if (first)
+            st.initial_residual = rho;
+          first = false;
+          state = control_check(rho, tol, its, max_steps);
+          if (state != 0)
+            break;
+          axpby(v0, 0.0, v0, 1.0 / rho, n);
+          std::fill(g.begin(), g.end(), 0.0);
+          g[0]  = rho;
+          int j = 0;
+          while (j < m)
+            {
+              ++its;
+              double *vj = ls.gm_v[j].get();
+              if (!right)
+                {
+                  vmult_fine(b, t, vj);
+                  precondition(b, w, t);
+                }
+              else
+                {
+                  precondition(b, t, vj);
+                  vmult_fine(b, w, t);
+                }
+              for (int i = 0; i <= j; ++i)
+                {
+                  dot_to(w, ls.gm_v[i].get(), n, ks + i);
+                  hipLaunchKernelGGL(axpy_slot_kernel, dim3(grid_for(n, 256)), dim3(256), 0,
+                                     stream, w, ls.gm_v[i].get(), ks, i, -1.0, n);
+                }
+              dot_to(w, w, n, ks + j + 1);
+              PPF_HIP_CHECK(hipMemcpyAsync(col.data(), ks, sizeof(double) * (j + 2),
+                                           hipMemcpyDeviceToHost, stream));
+              sync();
+              for (int i = 0; i <= j; ++i)
+                H[i][j] = col[i];
+              const double hn = std::sqrt(col[j + 1]);
+              H[j + 1][j]     = hn;
+              if (hn > 0.0 && j + 1 <= m)
+                axpby(ls.gm_v[j + 1].get(), 1.0 / hn, w, 0.0, n);
+              for (int i = 0; i < j; ++i)
+                {
+                  const double a = H[i][j], c = H[i + 1][j];
+                  H[i][j]        = cs[i] * a + sn[i] * c;
+                  H[i + 1][j]    = -sn[i] * a + cs[i] * c;
+                }
+              const double a = H[j][j], c = H[j + 1][j];
+              const double r = std::sqrt(a * a + c * c);
+              cs[j]          = r > 0.0 ? a / r : 1.0;
+              sn[j]          = r > 0.0 ? c / r : 0.0;
+              H[j][j]        = r;
+              H[j + 1][j]    = 0.0;
+              g[j + 1]       = -sn[j] * g[j];
+              g[j]           = cs[j] * g[j];
+              rho            = std::fabs(g[j + 1]);
+              ++j;
+              state = control_check(rho, tol, its, max_steps);
+              if (state != 0 || hn == 0.0)
+                break;
+            }
+          // x += V y (left) or P V y (right), R y = g over the j steps of this cycle.
+          for (int i = j - 1; i >= 0; --i)
+            {
+              double s = g[i];
+              for (int k = i + 1; k < j; ++k)
+                s -= H[i][k] * y[k];
+              y[i] = s / H[i][i];
+            }
+          if (!right)
+            for (int i = 0; i < j; ++i)
+              axpby(x, y[i], ls.gm_v[i].get(), 1.0, n);
+          else
+            {
+              fill(w, 0.0, n);
+              for (int i = 0; i < j; ++i)
+                axpby(w, y[i], ls.gm_v[i].get(), 1.0, n);
+              precondition(b, t, w);
+              axpby(x, 1.0, t, 1.0, n);
+            }
+          if (state != 0)
+            break;
+        }
+      st.iterations     = int(its);
+      st.final_residual = rho;
+      st.converged      = state > 0 ? 1 : 0;
+      return st;
     }
 
     /// deal.II SolverBicgstab (deal.II >= 9.6, lac/solver_bicgstab.h; the reference

@@ -50,6 +50,16 @@ struct LinearSolverParameters
   /// "0.000000", so the threshold is 0 unless a parameter file sets it.
   bool   bicgstab_exact_residual = true;
   double bicgstab_breakdown      = 0.0;
+  /// SolverRichardson::AdditionalData (solve_parameters.cc:191-210).
+  double richardson_omega                       = 1.0;
+  bool   richardson_use_preconditioned_residual = false;
+  /// SolverGMRES::AdditionalData (solve_parameters.cc:233-284).  Every orthogonalization
+  /// strategy name is accepted; this path orthogonalizes by modified Gram-Schmidt (the
+  /// strategies agree in exact arithmetic).
+  unsigned int gmres_max_basis_size       = 30;
+  bool         gmres_right_preconditioning = false;
+  bool         gmres_use_default_residual  = true;
+  std::string  gmres_orthogonalization     = "delayed_classical_gram_schmidt";
 
   // Settings of the MI355X path with no reference counterpart.
   LevelPrecision level_precision        = LevelPrecision::Double;

@@ -11,7 +11,7 @@
 //     (temporal_discretization.cc:35-107);
 //   * `boundary conditions: k` {variables, conditions} (constraint_parameters.cc:11-106);
 //   * `linear solver parameters: k` (solve_parameters.cc:10-340) including the
-//     `Chebyshev` and `BiCGStab` subsections;
+//     `Chebyshev`, `BiCGStab`, `Richardson` and `GMRES` subsections;
 //   * `newton solver parameters: k` (solve_parameters.cc:352-470);
 //   * `output` {condition, number, list} (io_parameters.cc:100-150);
 //   * `random seed` (miscellaneous_parameters.cc:11-48);
@@ -479,6 +479,25 @@ private:
         const std::string bs = p + "BiCGStab/";
         lp.bicgstab_exact_residual = prm::to_bool(f.get(bs, {"exact residual"}, "true"));
         lp.bicgstab_breakdown      = std::stod(f.get(bs, {"breakdown"}, "0.000000"));
+        const std::string rs = p + "Richardson/";
+        lp.richardson_omega  = std::stod(f.get(rs, {"omega"}, "1.0"));
+        lp.richardson_use_preconditioned_residual =
+          prm::to_bool(f.get(rs, {"use preconditioned residual"}, "false"));
+        const std::string gm = p + "GMRES/";
+        lp.gmres_max_basis_size = std::stoul(f.get(gm, {"max basis size"}, "30"));
+        if (lp.gmres_max_basis_size < 1)
+          throw std::invalid_argument("GMRES max basis size must be >= 1");
+        lp.gmres_right_preconditioning =
+          prm::to_bool(f.get(gm, {"right preconditioning"}, "false"));
+        lp.gmres_use_default_residual =
+          prm::to_bool(f.get(gm, {"use default residual"}, "true"));
+        lp.gmres_orthogonalization =
+          f.get(gm, {"orthogonalization strategy"}, "delayed_classical_gram_schmidt");
+        if (lp.gmres_orthogonalization != "classical_gram_schmidt" &&
+            lp.gmres_orthogonalization != "modified_gram_schmidt" &&
+            lp.gmres_orthogonalization != "delayed_classical_gram_schmidt")
+          throw std::invalid_argument("Invalid orthogonalization strategy: " +
+                                      lp.gmres_orthogonalization);
         static const std::set<std::string> solver_names = {
           "richardson", "cg", "bicgstab", "gmres", "fgmres", "minres"};
         if (!solver_names.count(lp.solver_type))

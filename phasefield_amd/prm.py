@@ -66,7 +66,8 @@ def allen_cahn(dim=2, size=100.0, refinement=8, steps=1000, dt=0.005, L=1.0, kap
 def allen_cahn_implicit(size=100.0, refinement=8, steps=10000, dt=1.0e-2, m_well=1.0, kappa=2.0,
                         linear_tolerance=1.0e-6, newton_tolerance=1.0e-6, newton_max_iterations=100,
                         step_size=1.0, preconditioner=None, subdivisions=None, outputs=5,
-                        periodic=False, tolerance_type="RMSEPerField", mg_depth=None):
+                        periodic=False, tolerance_type="RMSEPerField", mg_depth=None,
+                        solver="cg", linear_extra=()):
     """The implicit Allen-Cahn application's parameters (defaults: the reference's
     applications/allen_cahn/implicit/parameters.prm on a uniform mesh, 2-D)."""
     lines = [
@@ -76,6 +77,7 @@ def allen_cahn_implicit(size=100.0, refinement=8, steps=10000, dt=1.0e-2, m_well
         f"set Model constant kappa = {kappa!r}, DOUBLE",
         "subsection linear solver parameters: 0",
         "  set solver_ids = 1",
+        f"  set solver type = {solver}",
         "  set max iterations = 1000",
         f"  set tolerance type = {tolerance_type}",
         f"  set tolerance value = {linear_tolerance!r}",
@@ -84,6 +86,7 @@ def allen_cahn_implicit(size=100.0, refinement=8, steps=10000, dt=1.0e-2, m_well
         lines.append(f"  set preconditioner type = {preconditioner}")
         if mg_depth is not None:
             lines.append(f"  set mg depth = {mg_depth}")
+    lines += list(linear_extra)
     lines += [
         "end",
         "subsection newton solver parameters: 0",
@@ -101,7 +104,7 @@ def cahn_hilliard_implicit(size=100.0, refinement=7, steps=40000, dt=5.0e-4, McV
                            linear_tolerance=1.0e-6, newton_tolerance=1.0e-6,
                            newton_max_iterations=100, exact_residual=True, preconditioner=None,
                            subdivisions=None, outputs=500, periodic=False,
-                           tolerance_type="RMSEPerField"):
+                           tolerance_type="RMSEPerField", mg_depth=None, linear_extra=()):
     """The implicit Cahn-Hilliard application's parameters (defaults: the reference's
     applications/cahn_hilliard/implicit/parameters.prm on a uniform mesh, 2-D; end time
     20 = 40000 steps of 5e-4)."""
@@ -123,6 +126,9 @@ def cahn_hilliard_implicit(size=100.0, refinement=7, steps=40000, dt=5.0e-4, McV
     ]
     if preconditioner:
         lines.append(f"  set preconditioner type = {preconditioner}")
+        if mg_depth is not None:
+            lines.append(f"  set mg depth = {mg_depth}")
+    lines += list(linear_extra)
     lines += [
         "  subsection BiCGStab",
         f"    set exact residual = {'true' if exact_residual else 'false'}",

@@ -505,3 +505,98 @@ def test_time_dependent_dirichlet_values(oracle_libs, time_dependent):
     for f in (0, 1):
         u = o.get_field(f, -1)
         assert np.abs(u[mask] - g[mask]).max() <= 1e-13, f
+
+
+def test_newton_gmres_matches_dense_newton_and_scipy(oracle_libs):
+    """The oracle's GMRES restatement (left preconditioning, basis 30, no preconditioner)
+    inside the implicit Cahn-Hilliard Newton loop: same root and Newton count as a dense
+    Newton on the brute-force residual, and scipy's GMRES (restart 30) on the same scaled
+    systems, each started from the previous update, needs the same number of steps
+    within one per update plus 5 %."""
+    dt, Mc, Kc, W, tol, lin_tol = 0.05, 1.0, 1.5, 16.0, 1e-10, 1e-12
+    size = 16.0
+    text = prm.cahn_hilliard_implicit(size=size, refinement=2, steps=2, dt=dt, McV=Mc, KcV=Kc,
+                                      WcV=W, icamplitude=0.2, solver="gmres",
+                                      linear_tolerance=lin_tol, newton_tolerance=tol)
+    o = oracle("cahn_hilliard_implicit_2d_q2", text)
+    o.solve_increment()
+    c0 = o.get_field(0, -1)
+    o.solve_increment()
+    c1, mu1 = o.get_field(0, -1), o.get_field(1, 0)
+    ns = o.last_newton_stats(0)
+    mesh = bf.Mesh(2, 2, [4, 4], [size, size], [False, False])
+    K, mass = bf.assemble(mesh)
+    Kd, nu = K.toarray(), mass.size
+    cold = c0[mesh.unique]
+    Dinv = np.concatenate([1.0 / np.sqrt(mass)] * 2)
+
+    def residual(x):
+        c, mu = x[:nu], x[nu:]
+        return np.concatenate([mass * (cold - c) - dt * Mc * (Kd @ mu),
+                               mass * (mu - W * c * (c - 1.0) * (c - 0.5)) - Kc * (Kd @ c)])
+
+    def lhs(x):
+        c = x[:nu]
+        return np.block([[np.diag(mass), dt * Mc * Kd],
+                         [np.diag(mass * W * (0.5 + 3.0 * c * (c - 1.0))) + Kc * Kd, -np.diag(mass)]])
+
+    x = np.concatenate([cold, np.zeros(nu)])
+    du = np.zeros(2 * nu)
+    its, steps = 0, 0
+    while np.linalg.norm(Dinv * residual(x)) > tol * size * np.sqrt(2.0) and its < 50:
+        count = [0]
+        du, info = spla.gmres(Dinv[:, None] * lhs(x), Dinv * residual(x), x0=du, rtol=0.0,
+                              atol=lin_tol * size * np.sqrt(2.0), restart=30, maxiter=1000,
+                              callback=lambda r: count.__setitem__(0, count[0] + 1),
+                              callback_type="pr_norm")
+        assert info == 0
+        steps += count[0]
+        x = x + np.linalg.solve(lhs(x), residual(x))
+        its += 1
+    assert its >= 2 and ns["converged"] and ns["iterations"] == its
+    assert rel(c1[mesh.unique], x[:nu]) < 1e-9 and rel(mu1[mesh.unique], x[nu:]) < 1e-9
+    assert abs(ns["linear_iterations"] - steps) <= its + steps // 20, (ns, steps)
+
+
+def test_newton_richardson_matches_dense_newton_and_numpy(oracle_libs):
+    """The oracle's Richardson restatement (omega 0.065, no preconditioner) inside the
+    implicit Allen-Cahn Newton loop: same root and Newton count as a dense Newton, and
+    the same number of Richardson steps as x += omega (b - A x) on the assembled scaled
+    systems (each started from the previous update), within one per update."""
+    dt, m, kappa, tol, lin_tol, omega = 0.5, 1.0, 2.0, 1e-10, 1e-12, 0.065
+    text = prm.allen_cahn_implicit(refinement=3, steps=2, dt=dt, m_well=m, kappa=kappa,
+                                   linear_tolerance=lin_tol, newton_tolerance=tol,
+                                   solver="richardson",
+                                   linear_extra=["  subsection Richardson",
+                                                 f"    set omega = {omega}", "  end"])
+    o = oracle("allen_cahn_implicit_2d_q1", text)
+    o.solve_increment()
+    n0 = o.get_field(0, -1)
+    o.solve_increment()
+    n1 = o.get_field(0, -1)
+    ns = o.last_newton_stats(1)
+    mesh = bf.Mesh(2, 1, [8, 8], [100.0, 100.0], [False, False])
+    K, mass = bf.assemble(mesh)
+    Kd = K.toarray()
+    u_old = n0[mesh.unique]
+    D = 1.0 / np.sqrt(mass)
+
+    def residual(u):
+        return mass * (u_old - u - dt * m * 4.0 * u * (u - 1.0) * (u - 0.5)) - dt * kappa * m * (Kd @ u)
+
+    def lhs(u):
+        return np.diag(mass * (1.0 + dt * m * (12.0 * u * (u - 1.0) + 2.0))) + dt * kappa * m * Kd
+
+    u, du, its, steps = u_old.copy(), np.zeros_like(u_old), 0, 0
+    while np.linalg.norm(D * residual(u)) > tol * 100.0 and its < 50:
+        A, b = D[:, None] * lhs(u), D * residual(u)
+        k = 0
+        while np.linalg.norm(b - A @ du) > lin_tol * 100.0 and k < 1000:
+            du = du + omega * (b - A @ du)
+            k += 1
+        steps += k
+        u = u + np.linalg.solve(lhs(u), residual(u))
+        its += 1
+    assert its >= 3 and ns["converged"] and ns["iterations"] == its
+    assert rel(n1[mesh.unique], u) < 1e-9
+    assert abs(ns["linear_iterations"] - steps) <= its, (ns, steps)
