@@ -1372,14 +1372,17 @@ namespace oracle
     /// `max basis size` vectors, left (default) or right preconditioning, modified
     /// Gram-Schmidt, Givens rotations; each cycle checks the recomputed (preconditioned,
     /// for left preconditioning) residual at the accumulated step count and each Arnoldi
-    /// step the least-squares estimate.
+    /// step the least-squares estimate.  `flexible`: SolverFGMRES, right
+    /// preconditioning keeping z_j = P^{-1} v_j and updating x += Z y.
     template <class AOp, class POp>
     static Stats
     gmres(AOp &A, POp &P, std::vector<double> &x, const std::vector<double> &b, double tol,
-          unsigned max_steps, int m, bool right)
+          unsigned max_steps, int m, bool right, bool flexible = false)
     {
+      right                              = right || flexible;
       const std::size_t                n = x.size();
       std::vector<std::vector<double>> V(m + 1, std::vector<double>(n));
+      std::vector<std::vector<double>> Z(flexible ? m : 0, std::vector<double>(n));
       std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
       std::vector<double>              w(n), t(n), cs(m), sn(m), g(m + 1), y(m);
       Stats                            st;
@@ -1426,8 +1429,9 @@ namespace oracle
                 }
               else
                 {
-                  P(V[j], t);
-                  A(t, w);
+                  auto &z = flexible ? Z[j] : t;
+                  P(V[j], z);
+                  A(z, w);
                 }
               for (int i = 0; i <= j; ++i)
                 {
@@ -1468,11 +1472,12 @@ namespace oracle
                 s -= H[i][k] * y[k];
               y[i] = s / H[i][i];
             }
-          if (!right)
+          if (!right || flexible)
             {
+              const auto &B = flexible ? Z : V;
               for (int i = 0; i < j; ++i)
                 for (std::size_t k = 0; k < n; ++k)
-                  x[k] += y[i] * V[i][k];
+                  x[k] += y[i] * B[i][k];
             }
           else
             {
@@ -1546,15 +1551,15 @@ namespace oracle
       if (lp.solver_type == "bicgstab")
         return bicgstab(A, P, x, rhs, st.tol, lp.max_iterations, lp.bicgstab_exact_residual,
                         lp.bicgstab_breakdown);
-      if (lp.solver_type == "gmres")
+      if (lp.solver_type == "gmres" || lp.solver_type == "fgmres")
         return gmres(A, P, x, rhs, st.tol, lp.max_iterations, int(lp.gmres_max_basis_size),
-                     lp.gmres_right_preconditioning);
+                     lp.gmres_right_preconditioning, lp.solver_type == "fgmres");
       if (lp.solver_type == "richardson")
         return richardson(A, P, x, rhs, st.tol, lp.max_iterations, lp.richardson_omega,
                           lp.richardson_use_preconditioned_residual);
       if (lp.solver_type != "cg")
         throw std::invalid_argument("linear solver type " + lp.solver_type +
-                                    " is not part of the oracle (cg, bicgstab, gmres, richardson)");
+                                    " is not part of the oracle (cg, bicgstab, gmres, fgmres, richardson)");
       std::vector<double> g(n), h(n), d(n), ad(n);
       A(x, g);
       for (int64_t i = 0; i < n; ++i)

@@ -1262,7 +1262,7 @@ namespace ppf
       bool                 cheb_eigenvalues_estimated = false;
       DeviceBuffer<double> cheb_d, cheb_ax;
       DeviceBuffer<double> bs_r, bs_rbar, bs_p, bs_y, bs_z, bs_t, bs_v; // BiCGStab
-      std::vector<DeviceBuffer<double>> gm_v;                          // GMRES basis
+      std::vector<DeviceBuffer<double>> gm_v, gm_z;                    // GMRES bases
       DeviceBuffer<double>              gm_w, gm_t;
       std::vector<Level>   levels; // coarsest first
       SolveStats           stats;
@@ -1477,11 +1477,14 @@ namespace ppf
       if (type == "bicgstab")
         return bicgstab(b, x, rhs, tol, max_steps);
       if (type == "gmres")
-        return gmres(b, x, rhs, tol, max_steps);
+        return gmres(b, x, rhs, tol, max_steps, false);
+      if (type == "fgmres")
+        return gmres(b, x, rhs, tol, max_steps, true);
       if (type == "richardson")
         return richardson(b, x, rhs, tol, max_steps);
-      throw Error(status_unsupported, "linear solver type '" + type +
-                                        "' is not supported (cg, bicgstab, gmres, richardson)");
+      throw Error(status_unsupported,
+                  "linear solver type '" + type +
+                    "' is not supported (cg, bicgstab, gmres, fgmres, richardson)");
     }
 
     /// SolverControl::check: 1 success (value <= tol), -1 failure (step >= max_steps or
@@ -1538,20 +1541,32 @@ namespace ppf
     /// basis is orthogonalized by modified Gram-Schmidt with the inner products and
     /// updates on the device; the Hessenberg column (j + 2 values) is read once per step
     /// and the Givens rotations run on the host.
+    ///
+    /// `flexible`: deal.II's SolverFGMRES, right preconditioning that keeps the
+    /// preconditioned basis z_j = P^{-1} v_j and updates x += Z y (the same iterates as
+    /// right-preconditioned GMRES for a fixed preconditioner).
     SolveStats
-    gmres(int b, double *x, const double *rhs, double tol, unsigned int max_steps)
+    gmres(int b, double *x, const double *rhs, double tol, unsigned int max_steps,
+          bool flexible)
     {
       LinearState &ls = linear_state(b);
       const int64_t n = ls.n;
       const auto   &lp = lin_params(b);
-      if (!lp.gmres_use_default_residual)
+      if (!flexible && !lp.gmres_use_default_residual)
         throw Error(status_unsupported, "GMRES with use default residual = false");
       const int  m     = int(lp.gmres_max_basis_size);
-      const bool right = lp.gmres_right_preconditioning;
+      const bool right = flexible || lp.gmres_right_preconditioning;
       if (int(ls.gm_v.size()) != m + 1)
         ls.gm_v.resize(m + 1);
       for (auto &v : ls.gm_v)
         v.resize(n);
+      if (flexible)
+        {
+          if (int(ls.gm_z.size()) != m)
+            ls.gm_z.resize(m);
+          for (auto &z : ls.gm_z)
+            z.resize(n);
+        }
       ls.gm_w.resize(n);
       ls.gm_t.resize(n);
       krylov_slots.resize(std::max<int64_t>(8, m + 2));
@@ -1601,8 +1616,9 @@ namespace ppf
                 }
               else
                 {
-                  precondition(b, t, vj);
-                  vmult_fine(b, w, t);
+                  double *z = flexible ? ls.gm_z[j].get() : t;
+                  precondition(b, z, vj);
+                  vmult_fine(b, w, z);
                 }
               for (int i = 0; i <= j; ++i)
                 {
@@ -1651,6 +1667,9 @@ namespace ppf
           if (!right)
             for (int i = 0; i < j; ++i)
               axpby(x, y[i], ls.gm_v[i].get(), 1.0, n);
+          else if (flexible)
+            for (int i = 0; i < j; ++i)
+              axpby(x, y[i], ls.gm_z[i].get(), 1.0, n);
           else
             {
               fill(w, 0.0, n);

@@ -6,7 +6,8 @@ Per increment: the same number of Newton updates, Krylov steps within 15 % (thei
 at a tight tolerance moves with the last bits of the dot products, which the two sides
 sum in different orders), and every field within 1e-9 relative.  Cases: GMRES with left
 preconditioning (none, Chebyshev) and right preconditioning (GMG) on the two-field
-implicit Cahn-Hilliard application, a GMRES basis small enough to restart, and
+implicit Cahn-Hilliard application, a GMRES basis small enough to restart, flexible
+GMRES (SolverFGMRES) with GMG, and
 Richardson (unpreconditioned with a damping factor, and Chebyshev-preconditioned) on the
 implicit Allen-Cahn application.
 """
@@ -40,6 +41,9 @@ CASES = {
         refinement=5, steps=2, dt=1e-3, solver="gmres", linear_tolerance=1e-12,
         newton_tolerance=1e-10,
         linear_extra=["  subsection GMRES", "    set max basis size = 8", "  end"])),
+    "fgmres_gmg": (CH, 0, prm.cahn_hilliard_implicit(
+        refinement=5, steps=3, dt=1e-3, solver="fgmres", linear_tolerance=1e-12,
+        newton_tolerance=1e-10, preconditioner="GMG", mg_depth=3)),
     "richardson_damped": (AC, 1, prm.allen_cahn_implicit(
         refinement=3, steps=3, dt=0.5, solver="richardson", linear_tolerance=1e-12,
         newton_tolerance=1e-10,

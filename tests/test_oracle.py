@@ -600,3 +600,23 @@ def test_newton_richardson_matches_dense_newton_and_numpy(oracle_libs):
     assert its >= 3 and ns["converged"] and ns["iterations"] == its
     assert rel(n1[mesh.unique], u) < 1e-9
     assert abs(ns["linear_iterations"] - steps) <= its, (ns, steps)
+
+
+def test_fgmres_equals_right_preconditioned_gmres(oracle_libs):
+    """With a fixed preconditioner, flexible GMRES (right preconditioning keeping the
+    preconditioned basis) produces the iterates of right-preconditioned GMRES: same
+    Newton and Krylov step counts and the same implicit Cahn-Hilliard state."""
+    right = ["  subsection GMRES", "    set right preconditioning = true", "  end"]
+    out = {}
+    for solver, extra in (("fgmres", ()), ("gmres", right)):
+        text = prm.cahn_hilliard_implicit(refinement=4, steps=2, dt=1e-3, solver=solver,
+                                          preconditioner="Chebyshev", linear_tolerance=1e-12,
+                                          newton_tolerance=1e-10, linear_extra=extra)
+        o = oracle("cahn_hilliard_implicit_2d_q2", text)
+        o.solve_increment()
+        o.solve_increment()
+        out[solver] = (o.last_newton_stats(0), o.get_field(0, -1), o.get_field(1, 0))
+    a, b = out["fgmres"], out["gmres"]
+    assert a[0]["iterations"] == b[0]["iterations"]
+    assert a[0]["linear_iterations"] == b[0]["linear_iterations"]
+    assert rel(a[1], b[1]) < 1e-12 and rel(a[2], b[2]) < 1e-10
