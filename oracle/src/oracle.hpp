@@ -1368,6 +1368,86 @@ namespace oracle
       return st;
     }
 
+    /// deal.II SolverMinRes (lac/solver_minres.h), restated as the published
+    /// Paige-Saunders MINRES (SIAM J. Numer. Anal. 12, 1975): preconditioned Lanczos
+    /// three-term recurrence, one Givens rotation per step, direction vectors
+    /// d_k = (v_k - eps_k d_{k-2} - delta_k d_{k-1}) / gamma_k.  The checked value is
+    /// the residual estimate phibar (|r| in the P^{-1} norm; |b - A x| without a
+    /// preconditioner) at step 0 and after each step.
+    template <class AOp, class POp>
+    static Stats
+    minres(AOp &A, POp &P, std::vector<double> &x, const std::vector<double> &b, double tol,
+           unsigned max_steps)
+    {
+      const std::size_t   n = x.size();
+      std::vector<double> u_prev(n, 0.0), u_cur(n), z(n), q(n), Aq(n);
+      std::vector<double> d_prev2(n, 0.0), d_prev(n, 0.0), d(n);
+      Stats               st;
+      st.tol = tol;
+      A(x, Aq);
+      for (std::size_t i = 0; i < n; ++i)
+        u_cur[i] = b[i] - Aq[i];
+      P(u_cur, z);
+      double beta_k = std::sqrt(std::max(0.0, dot(u_cur, z)));
+      double beta_prev = 0.0;
+      double phibar    = beta_k;
+      st.r0            = phibar;
+      // Rotation of the previous two steps: (c_{k-1}, s_{k-1}) and the pieces of the
+      // tridiagonal column they leave behind.
+      double   c = -1.0, s = 0.0, dbar = 0.0, eps_next = 0.0;
+      unsigned step = 0;
+      int      state = phibar <= tol ? 1 : (step >= max_steps ? -1 : 0);
+      while (state == 0)
+        {
+          ++step;
+          for (std::size_t i = 0; i < n; ++i)
+            q[i] = z[i] / beta_k;
+          A(q, Aq);
+          // Next Lanczos vector (unpreconditioned form): u = A q - beta_k u_{k-1}/beta_{k-1}
+          // - alpha u_k/beta_k, alpha = q.(A q - beta_k u_{k-1}/beta_{k-1}) (the modified
+          // Gram-Schmidt order of the published algorithm).
+          if (step >= 2)
+            for (std::size_t i = 0; i < n; ++i)
+              Aq[i] -= (beta_k / beta_prev) * u_prev[i];
+          const double alpha = dot(q, Aq);
+          for (std::size_t i = 0; i < n; ++i)
+            {
+              u_prev[i] = u_cur[i];
+              u_cur[i]  = Aq[i] - (alpha / beta_k) * u_cur[i];
+            }
+          P(u_cur, z);
+          beta_prev               = beta_k;
+          beta_k                  = std::sqrt(std::max(0.0, dot(u_cur, z)));
+          const double eps_k      = eps_next;
+          const double delta_k    = c * dbar + s * alpha;
+          const double gamma_bar  = s * dbar - c * alpha;
+          eps_next                = s * beta_k;
+          dbar                    = -c * beta_k;
+          const double gamma_k    = std::max(std::hypot(gamma_bar, beta_k),
+                                          std::numeric_limits<double>::epsilon());
+          c                       = gamma_bar / gamma_k;
+          s                       = beta_k / gamma_k;
+          const double tau        = c * phibar;
+          phibar                  = s * phibar;
+          for (std::size_t i = 0; i < n; ++i)
+            {
+              d[i] = (q[i] - eps_k * d_prev2[i] - delta_k * d_prev[i]) / gamma_k;
+              x[i] += tau * d[i];
+            }
+          std::swap(d_prev2, d_prev);
+          std::swap(d_prev, d);
+          if (!std::isfinite(phibar))
+            throw std::runtime_error("MINRES produced a non-finite residual");
+          state = phibar <= tol ? 1 : (step >= max_steps ? -1 : 0);
+          if (state == 0 && beta_k == 0.0)
+            state = -1;
+        }
+      st.iterations = int(step);
+      st.r          = phibar;
+      st.converged  = state > 0;
+      return st;
+    }
+
     /// deal.II SolverGMRES (lac/solver_gmres.h), restated: restarted GMRES, basis of
     /// `max basis size` vectors, left (default) or right preconditioning, modified
     /// Gram-Schmidt, Givens rotations; each cycle checks the recomputed (preconditioned,
@@ -1557,9 +1637,11 @@ namespace oracle
       if (lp.solver_type == "richardson")
         return richardson(A, P, x, rhs, st.tol, lp.max_iterations, lp.richardson_omega,
                           lp.richardson_use_preconditioned_residual);
+      if (lp.solver_type == "minres")
+        return minres(A, P, x, rhs, st.tol, lp.max_iterations);
       if (lp.solver_type != "cg")
         throw std::invalid_argument("linear solver type " + lp.solver_type +
-                                    " is not part of the oracle (cg, bicgstab, gmres, fgmres, richardson)");
+                                    " is not part of the oracle");
       std::vector<double> g(n), h(n), d(n), ad(n);
       A(x, g);
       for (int64_t i = 0; i < n; ++i)

@@ -1467,7 +1467,8 @@ namespace ppf
     }
 
     /// The block's linear solver (SolverSelector by `solver type`, reference
-    /// linear_solver.h:461-490): CG or BiCGStab on this path.
+    /// linear_solver.h:461-490): every name the parser accepts (cg, bicgstab, gmres,
+    /// fgmres, richardson, minres).
     SolveStats
     krylov(int b, double *x, const double *rhs, double tol, unsigned int max_steps)
     {
@@ -1482,9 +1483,84 @@ namespace ppf
         return gmres(b, x, rhs, tol, max_steps, true);
       if (type == "richardson")
         return richardson(b, x, rhs, tol, max_steps);
-      throw Error(status_unsupported,
-                  "linear solver type '" + type +
-                    "' is not supported (cg, bicgstab, gmres, fgmres, richardson)");
+      if (type == "minres")
+        return minres(b, x, rhs, tol, max_steps);
+      throw Error(status_unsupported, "unknown linear solver type '" + type + "'");
+    }
+
+    /// MINRES for a symmetric operator with a symmetric positive definite
+    /// preconditioner (deal.II's SolverMinRes, lac/solver_minres.h; `solver type =
+    /// minres`), restated as the published Paige-Saunders recurrence (Lanczos on
+    /// P^{-1} A, Givens rotations, three-term direction update).  The checked residual is
+    /// the recurrence's estimate phibar, |r|_P (|b - A x| without a preconditioner),
+    /// at step 0 and after every step.
+    SolveStats
+    minres(int b, double *x, const double *rhs, double tol, unsigned int max_steps)
+    {
+      LinearState &ls = linear_state(b);
+      const int64_t n = ls.n;
+      for (auto *buf : {&ls.bs_r, &ls.bs_rbar, &ls.bs_p, &ls.bs_y, &ls.bs_z, &ls.bs_t, &ls.bs_v})
+        buf->resize(n);
+      double *r1 = ls.bs_r.get(), *r2 = ls.bs_rbar.get(), *y = ls.bs_y.get(), *v = ls.bs_v.get(),
+             *w = ls.bs_p.get(), *w1 = ls.bs_z.get(), *w2 = ls.bs_t.get();
+      SolveStats st;
+      st.tolerance = tol;
+      vmult_fine(b, r1, x);
+      axpby(r1, 1.0, rhs, -1.0, n); // r1 = b - A x
+      precondition(b, y, r1);
+      const double beta1 = std::sqrt(std::max(0.0, dot(r1, y, n)));
+      double       phibar = beta1;
+      st.initial_residual = phibar;
+      unsigned step  = 0;
+      int      state = control_check(phibar, tol, step, max_steps);
+      if (state == 0)
+        {
+          copy(r2, r1, n);
+          fill(w, 0.0, n);
+          fill(w2, 0.0, n);
+          double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, cs = -1.0, sn = 0.0;
+          while (state == 0)
+            {
+              ++step;
+              axpby(v, 1.0 / beta, y, 0.0, n); // v = y / beta
+              vmult_fine(b, y, v);
+              if (step >= 2)
+                axpby(y, -beta / oldb, r1, 1.0, n);
+              const double alfa = dot(v, y, n);
+              axpby(y, -alfa / beta, r2, 1.0, n);
+              std::swap(r1, r2); // r1 = old r2
+              copy(r2, y, n);
+              precondition(b, y, r2);
+              oldb                = beta;
+              beta                = std::sqrt(std::max(0.0, dot(r2, y, n)));
+              const double oldeps = epsln;
+              const double delta  = cs * dbar + sn * alfa;
+              const double gbar   = sn * dbar - cs * alfa;
+              epsln               = sn * beta;
+              dbar                = -cs * beta;
+              const double gamma =
+                std::max(std::sqrt(gbar * gbar + beta * beta), std::numeric_limits<double>::epsilon());
+              cs               = gbar / gamma;
+              sn               = beta / gamma;
+              const double phi = cs * phibar;
+              phibar           = sn * phibar;
+              // w = (v - oldeps w1 - delta w2) / gamma, with (w1, w2) <- (w2, w).
+              std::swap(w1, w2);
+              std::swap(w2, w);
+              copy(w, v, n);
+              axpby(w, -oldeps, w1, 1.0, n);
+              axpby(w, -delta, w2, 1.0, n);
+              axpby(w, 0.0, w, 1.0 / gamma, n);
+              axpby(x, phi, w, 1.0, n);
+              state = control_check(phibar, tol, step, max_steps);
+              if (beta == 0.0 && state == 0)
+                state = phibar <= tol ? 1 : -1;
+            }
+        }
+      st.iterations     = int(step);
+      st.final_residual = phibar;
+      st.converged      = state > 0 ? 1 : 0;
+      return st;
     }
 
     /// SolverControl::check: 1 success (value <= tol), -1 failure (step >= max_steps or

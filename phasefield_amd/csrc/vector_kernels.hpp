@@ -66,8 +66,9 @@ namespace ppf
   axpby_kernel(T *__restrict__ y, T a, const T *__restrict__ x, T b, int64_t n)
   {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    // b = 0 overwrites y without reading it (y may hold anything, NaN included).
     if (i < n)
-      y[i] = a * x[i] + b * y[i];
+      y[i] = b == T(0) ? a * x[i] : a * x[i] + b * y[i];
   }
 
   /// z = x - y

@@ -1,5 +1,5 @@
-"""GMRES and Richardson inside Newton blocks on the GPU against the oracle (deal.II's
-SolverGMRES and SolverRichardson as the reference selects them by `solver type`,
+"""GMRES, Richardson and MINRES inside Newton blocks on the GPU against the oracle (deal.II's
+SolverGMRES, SolverRichardson and SolverMinRes as the reference selects them by `solver type`,
 include/prismspf/solvers/linear_solver.h:461-490, restated on both sides).
 
 Per increment: the same number of Newton updates, Krylov steps within 15 % (their count
@@ -7,9 +7,11 @@ at a tight tolerance moves with the last bits of the dot products, which the two
 sum in different orders), and every field within 1e-9 relative.  Cases: GMRES with left
 preconditioning (none, Chebyshev) and right preconditioning (GMG) on the two-field
 implicit Cahn-Hilliard application, a GMRES basis small enough to restart, flexible
-GMRES (SolverFGMRES) with GMG, and
-Richardson (unpreconditioned with a damping factor, and Chebyshev-preconditioned) on the
-implicit Allen-Cahn application.
+GMRES (SolverFGMRES) with GMG,
+Richardson (unpreconditioned with a damping factor, and Chebyshev-preconditioned) and
+MINRES (SolverMinRes; unpreconditioned and Chebyshev-preconditioned, absolute-residual
+tolerance so that the Newton system is the unscaled symmetric Jacobian) on the implicit
+Allen-Cahn application.
 """
 
 from __future__ import annotations
@@ -51,6 +53,12 @@ CASES = {
     "richardson_chebyshev": (AC, 1, prm.allen_cahn_implicit(
         refinement=5, steps=3, dt=0.5, solver="richardson", linear_tolerance=1e-12,
         newton_tolerance=1e-10, preconditioner="Chebyshev")),
+    "minres_none": (AC, 1, prm.allen_cahn_implicit(
+        refinement=5, steps=3, dt=0.5, solver="minres", linear_tolerance=1e-11,
+        newton_tolerance=1e-9, tolerance_type="AbsoluteResidual")),
+    "minres_chebyshev": (AC, 1, prm.allen_cahn_implicit(
+        refinement=5, steps=3, dt=0.5, solver="minres", linear_tolerance=1e-11,
+        newton_tolerance=1e-9, tolerance_type="AbsoluteResidual", preconditioner="Chebyshev")),
 }
 
 

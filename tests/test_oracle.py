@@ -620,3 +620,59 @@ def test_fgmres_equals_right_preconditioned_gmres(oracle_libs):
     assert a[0]["iterations"] == b[0]["iterations"]
     assert a[0]["linear_iterations"] == b[0]["linear_iterations"]
     assert rel(a[1], b[1]) < 1e-12 and rel(a[2], b[2]) < 1e-10
+
+
+def test_newton_minres_matches_dense_newton_and_scipy(oracle_libs):
+    """The oracle's MINRES restatement (no preconditioner, absolute residual: the
+    unscaled symmetric Jacobian) inside the implicit Allen-Cahn Newton loop: same root and
+    Newton count as a dense Newton, and per update the first step at which scipy's MINRES
+    iterate (same system, started from the previous update) has |b - A x| <= tol, within
+    one per update."""
+    dt, m, kappa, tol, lin_tol = 0.5, 1.0, 2.0, 1e-9, 1e-11
+    text = prm.allen_cahn_implicit(refinement=3, steps=2, dt=dt, m_well=m, kappa=kappa,
+                                   linear_tolerance=lin_tol, newton_tolerance=tol,
+                                   solver="minres", tolerance_type="AbsoluteResidual")
+    o = oracle("allen_cahn_implicit_2d_q1", text)
+    o.solve_increment()
+    n0 = o.get_field(0, -1)
+    o.solve_increment()
+    n1 = o.get_field(0, -1)
+    ns = o.last_newton_stats(1)
+    mesh = bf.Mesh(2, 1, [8, 8], [100.0, 100.0], [False, False])
+    K, mass = bf.assemble(mesh)
+    Kd = K.toarray()
+    u_old = n0[mesh.unique]
+
+    def residual(u):
+        return mass * (u_old - u - dt * m * 4.0 * u * (u - 1.0) * (u - 0.5)) - dt * kappa * m * (Kd @ u)
+
+    def lhs(u):
+        return np.diag(mass * (1.0 + dt * m * (12.0 * u * (u - 1.0) + 2.0))) + dt * kappa * m * Kd
+
+    class Reached(Exception):
+        pass
+
+    u, du, its, steps = u_old.copy(), np.zeros_like(u_old), 0, 0
+    while np.linalg.norm(residual(u)) > tol and its < 50:
+        A, b = lhs(u), residual(u)
+        k, x_last = 0, du
+        if np.linalg.norm(b - A @ du) > lin_tol:
+            count = [0]
+
+            def cb(xk):
+                count[0] += 1
+                if np.linalg.norm(b - A @ xk) <= lin_tol:
+                    x_last_holder[0] = xk.copy()
+                    raise Reached
+
+            x_last_holder = [None]
+            with pytest.raises(Reached):
+                spla.minres(A, b, x0=du, rtol=1e-30, maxiter=1000, callback=cb)
+            k, x_last = count[0], x_last_holder[0]
+        du = x_last
+        steps += k
+        u = u + np.linalg.solve(lhs(u), residual(u))
+        its += 1
+    assert its >= 3 and ns["converged"] and ns["iterations"] == its
+    assert rel(n1[mesh.unique], u) < 1e-9
+    assert abs(ns["linear_iterations"] - steps) <= its, (ns, steps)
