@@ -12,7 +12,8 @@
  * (degree*cells_d + 1) nodes per direction, periodic images included, lexicographic
  * with x fastest.  Device arrays passed to ppf_apply_operator use the stored layout
  * reported by ppf_get_info (periodic images omitted), one block after another for
- * multi-field solve blocks.
+ * multi-field solve blocks.  A vector field appears as dim component fields named
+ * <name>_x, <name>_y, <name>_z (counted in ppf_info.n_fields).
  *
  * Each entry point names the reference interface it replaces (paths relative to the
  * PRISMS-PF source tree).

@@ -27,6 +27,7 @@ APPS = [
     "allen_cahn_3d_q1",
     "cahn_hilliard_3d_q2",
     "cahn_hilliard_implicit_2d_q2",
+    "mechanics_3d_q2",
     "poisson_2d_q2",
     "poisson_3d_q2",
     "precipitate_3d_q1",

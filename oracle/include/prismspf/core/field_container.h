@@ -22,9 +22,10 @@ public:
   using ScalarValue = dealii::VectorizedArray<number>;
   using VectorValue = dealii::Tensor<1, dim, ScalarValue>;
   template <TensorRank Rank>
-  using Value = ScalarValue;
+  using Value = std::conditional_t<Rank == Vector, VectorValue, ScalarValue>;
   template <TensorRank Rank>
-  using Gradient = dealii::Tensor<1, dim, ScalarValue>;
+  using Gradient = std::conditional_t<Rank == Vector, dealii::Tensor<2, dim, ScalarValue>,
+                                      dealii::Tensor<1, dim, ScalarValue>>;
 
   static constexpr unsigned int n_q_points =
     dim == 2 ? (degree + 1) * (degree + 1) : (degree + 1) * (degree + 1) * (degree + 1);
@@ -37,16 +38,31 @@ public:
     return int(t) + 1;
   }
 
+  // A vector field occupies dim consecutive stored fields, component c at f + c.
   template <TensorRank Rank, DependencyType type>
-  ScalarValue
+  Value<Rank>
   get_value(Types::Index f) const
   {
     return get_value<Rank>(f, type);
   }
 
   template <TensorRank Rank>
-  ScalarValue
+  Value<Rank>
   get_value(Types::Index f, DependencyType type) const
+  {
+    if constexpr (Rank == Vector)
+      {
+        VectorValue v;
+        for (unsigned int c = 0; c < dim; ++c)
+          v[c] = get_value<Scalar>(f + c, type);
+        return v;
+      }
+    else
+      return component_value(f, type);
+  }
+
+  ScalarValue
+  component_value(Types::Index f, DependencyType type) const
   {
     const int s = slot_of(type);
     if (!(avail_values[f][s]))
@@ -65,13 +81,36 @@ public:
   Gradient<Rank>
   get_gradient(Types::Index f, DependencyType type) const
   {
-    const int s = slot_of(type);
-    if (!(avail_gradients[f][s]))
-      throw std::logic_error("gradient of an undeclared dependency was read");
-    Gradient<Rank> g;
-    for (unsigned int d = 0; d < dim; ++d)
-      g[d] = gradients[f][s][d];
-    return g;
+    if constexpr (Rank == Vector)
+      {
+        Gradient<Rank> g; // g[c][d] = d u_c / d x_d
+        for (unsigned int c = 0; c < dim; ++c)
+          g[c] = get_gradient<Scalar>(f + c, type);
+        return g;
+      }
+    else
+      {
+        const int s = slot_of(type);
+        if (!(avail_gradients[f][s]))
+          throw std::logic_error("gradient of an undeclared dependency was read");
+        Gradient<Rank> g;
+        for (unsigned int d = 0; d < dim; ++d)
+          g[d] = gradients[f][s][d];
+        return g;
+      }
+  }
+
+  template <TensorRank Rank, DependencyType type>
+  Gradient<Rank>
+  get_symmetric_gradient(Types::Index f) const
+  {
+    static_assert(Rank == Vector, "get_symmetric_gradient needs a vector field");
+    const Gradient<Rank> g = get_gradient<Rank>(f, type);
+    Gradient<Rank>       e;
+    for (unsigned int i = 0; i < dim; ++i)
+      for (unsigned int j = 0; j < dim; ++j)
+        e[i][j] = ScalarValue(0.5 * (g[i][j].v + g[j][i].v));
+    return e;
   }
 
   dealii::Point<dim, ScalarValue>
@@ -105,7 +144,12 @@ public:
   {
     if (!(writable & (1u << f)))
       throw std::logic_error("submission for a field not part of this solve block");
-    if constexpr (dealii::is_va<ValType>::value)
+    if constexpr (dealii::is_tensor<ValType>::value)
+      {
+        for (unsigned int c = 0; c < dim; ++c)
+          set_value_term(f + c, val[c]);
+      }
+    else if constexpr (dealii::is_va<ValType>::value)
       value_term[f] = val.v;
     else
       value_term[f] = number(val);
@@ -114,6 +158,19 @@ public:
   template <typename GradType>
   void
   set_gradient_term(Types::Index f, const GradType &val)
+  {
+    if constexpr (GradType::rank == 2)
+      {
+        for (unsigned int c = 0; c < dim; ++c)
+          set_gradient_term(f + c, val[c]);
+      }
+    else
+      set_component_gradient_term(f, val);
+  }
+
+  template <typename GradType>
+  void
+  set_component_gradient_term(Types::Index f, const GradType &val)
   {
     if (!(writable & (1u << f)))
       throw std::logic_error("submission for a field not part of this solve block");

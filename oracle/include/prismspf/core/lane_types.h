@@ -107,6 +107,16 @@ namespace dealii
   template <int dim, typename N>
   N operator*(const Tensor<1, dim, N> &a, const Tensor<1, dim, N> &b) { N s = a[0] * b[0]; for (int i = 1; i < dim; ++i) s += a[i] * b[i]; return s; }
 
+  template <int dim, typename Number>
+  struct Tensor<2, dim, Number>
+  {
+    static constexpr unsigned int rank = 2;
+    using value_type = Number;
+    Tensor<1, dim, Number> r[dim];
+    Tensor<1, dim, Number> &operator[](unsigned i) { return r[i]; }
+    const Tensor<1, dim, Number> &operator[](unsigned i) const { return r[i]; }
+  };
+
   template <int dim, typename Number = double>
   struct Point : Tensor<1, dim, Number>
   {

@@ -203,6 +203,8 @@ namespace oracle
           if (it != inputs.boundary_parameters.end())
             fa.boundary_conditions = it->second;
         }
+      // A vector field is stored as dim scalar component fields (declared last).
+      layout = prismspf::expand_vector_fields(fields, blocks, dim);
       pde   = std::make_unique<PDE>(inputs, tools);
       timer = prismspf::SimulationTimer(inputs.temporal_discretization.dt,
                                         inputs.temporal_discretization.initial_time);
@@ -309,9 +311,11 @@ namespace oracle
                   dealii::Point<dim> pt;
                   for (int e = 0; e < dim; ++e)
                     pt[e] = m.lower[e] + i[e] * m.h[e] / degree;
-                  double v = 0.0, vec = 0.0;
-                  hooks().set_dirichlet(unsigned(f), bid, 0u, pt, timer, v, vec);
-                  c.g[dof] = v;
+                  double      v = 0.0, vec = 0.0;
+                  const auto &lc = layout[f];
+                  hooks().set_dirichlet(unsigned(lc.declared), bid, lc.component, pt, timer, v,
+                                        vec);
+                  c.g[dof] = lc.vector ? vec : v;
                 }
               break;
             }
@@ -1794,9 +1798,10 @@ namespace oracle
               dealii::Point<dim> p;
               for (int d = 0; d < dim; ++d)
                 p[d] = top.lower[d] + i[d] * top.h[d] / degree;
-              double val = 0.0, vec = 0.0;
-              hooks().set_initial_condition(unsigned(f), 0u, p, val, vec);
-              v[dof] = val;
+              double      val = 0.0, vec = 0.0;
+              const auto &lc  = layout[f];
+              hooks().set_initial_condition(unsigned(lc.declared), lc.component, p, val, vec);
+              v[dof] = lc.vector ? vec : val;
             }
         }
     }
@@ -2008,6 +2013,7 @@ namespace oracle
     prismspf::PhaseFieldTools<dim>                   tools;
     Basis1D                                          basis;
     std::vector<prismspf::FieldAttributes>           fields;
+    std::vector<prismspf::StoredComponent>           layout;
     std::vector<SolveBlock>                          blocks;
     std::unique_ptr<PDE>                             pde;
     prismspf::SimulationTimer                        timer;

@@ -27,6 +27,7 @@ APPS = {
     "allen_cahn_3d_q1": "apps/allen_cahn/app_3d.hip",
     "cahn_hilliard_3d_q2": "apps/cahn_hilliard/app.hip",
     "cahn_hilliard_implicit_2d_q2": "apps/cahn_hilliard_implicit/app_2d.hip",
+    "mechanics_3d_q2": "apps/mechanics/app_3d.hip",
     "poisson_2d_q2": "apps/poisson/app_2d.hip",
     "poisson_3d_q2": "apps/poisson/app_3d.hip",
     "precipitate_3d_q1": "apps/precipitate/app.hip",

@@ -147,15 +147,6 @@ namespace ppf
     {
       App::define(fields, blocks);
       prismspf::validate_solve_blocks(blocks, fields);
-      if (int(fields.size()) != App::n_fields)
-        throw Error(status_internal, "App::n_fields does not match the declared fields");
-      for (const auto &fa : fields)
-        if (fa.field_type != prismspf::Scalar)
-          throw Error(status_unsupported, "field " + fa.name +
-                                            ": only scalar fields are supported on the "
-                                            "MI355X path");
-      if (fields.size() > PPF_MAX_FIELDS)
-        throw Error(status_unsupported, "more than PPF_MAX_FIELDS fields");
 
       // Parameter-file overrides of the declared solver settings and boundary
       // conditions (reference problem.cc:152-224).
@@ -174,6 +165,18 @@ namespace ppf
           if (it != inputs.boundary_parameters.end())
             fa.boundary_conditions = it->second;
         }
+      try
+        {
+          layout = prismspf::expand_vector_fields(fields, blocks, dim);
+        }
+      catch (const std::invalid_argument &e)
+        {
+          throw Error(status_unsupported, e.what());
+        }
+      if (int(fields.size()) != App::n_fields)
+        throw Error(status_internal, "App::n_fields does not match the stored fields");
+      if (fields.size() > PPF_MAX_FIELDS)
+        throw Error(status_unsupported, "more than PPF_MAX_FIELDS fields");
 
       pde_d = std::make_unique<PDE<double>>(inputs, tools);
       pde_f = std::make_unique<PDE<float>>(inputs, tools);
@@ -330,9 +333,10 @@ namespace ppf
       dealii::Point<dim> p;
       for (int d = 0; d < dim; ++d)
         p[d] = fine.lower[d] + i[d] * fine.h[d] / degree;
-      double value = 0.0, vec = 0.0;
-      pde_d->set_dirichlet(unsigned(f), bid, 0u, p, timer, value, vec);
-      return value;
+      double     value = 0.0, vec = 0.0;
+      const auto &lc    = layout[f];
+      pde_d->set_dirichlet(unsigned(lc.declared), bid, lc.component, p, timer, value, vec);
+      return lc.vector ? vec : value;
     }
 
     bool
@@ -2515,9 +2519,10 @@ namespace ppf
               dealii::Point<dim> p;
               for (int d = 0; d < dim; ++d)
                 p[d] = fine.lower[d] + i[d] * fine.h[d] / degree;
-              double value = 0.0, vec = 0.0;
-              pde_d->set_initial_condition(f, 0u, p, value, vec);
-              host[node] = value;
+              double     value = 0.0, vec = 0.0;
+              const auto &lc    = layout[f];
+              pde_d->set_initial_condition(unsigned(lc.declared), lc.component, p, value, vec);
+              host[node] = lc.vector ? vec : value;
             }
           PPF_HIP_CHECK(hipMemcpyAsync(field_ptr(int(f), 0), host.data(),
                                        sizeof(double) * fine.n_nodes, hipMemcpyHostToDevice,
@@ -2841,6 +2846,7 @@ namespace ppf
     prismspf::UserInputParameters<dim>    inputs;
     prismspf::PhaseFieldTools<dim>        tools;
     std::vector<prismspf::FieldAttributes> fields;
+    std::vector<prismspf::StoredComponent>  layout;
     std::vector<prismspf::SolveBlock>      blocks;
     std::unique_ptr<PDE<double>>           pde_d;
     std::unique_ptr<PDE<float>>            pde_f;

@@ -53,16 +53,74 @@ public:
   [[nodiscard]] PPF_HD PPF_INLINE Value<Rank>
   get_value(Types::Index field_index) const
   {
-    static_assert(Rank == Scalar, "Only scalar fields are supported on the MI355X path");
+    static_assert(Rank == Scalar || Rank == Vector, "Unknown field rank");
     static_assert(type != DST, "The destination of a solve cannot be read");
     return get_value<Rank>(field_index, type);
   }
 
+  // A vector field is stored as dim component fields starting at its own index
+  // (field_index + c holds component c), so its accessors gather the components.
   template <TensorRank Rank>
   [[nodiscard]] PPF_HD PPF_INLINE Value<Rank>
   get_value(Types::Index field_index, DependencyType type) const
   {
-    static_assert(Rank == Scalar, "Only scalar fields are supported on the MI355X path");
+    static_assert(Rank == Scalar || Rank == Vector, "Unknown field rank");
+    if constexpr (Rank == Vector && dim > 1)
+      {
+        Value<Rank> v;
+        for (unsigned int c = 0; c < dim; ++c)
+          v[c] = get_value<Scalar>(field_index + c, type);
+        return v;
+      }
+    else
+      return scalar_value(field_index, type);
+  }
+
+  template <TensorRank Rank, DependencyType type>
+  [[nodiscard]] PPF_HD PPF_INLINE Gradient<Rank>
+  get_gradient(Types::Index field_index) const
+  {
+    static_assert(Rank == Scalar || Rank == Vector, "Unknown field rank");
+    static_assert(type != DST, "The destination of a solve cannot be read");
+    return get_gradient<Rank>(field_index, type);
+  }
+
+  /// Gradient; for a vector field g[c][d] = d u_c / d x_d.
+  template <TensorRank Rank>
+  [[nodiscard]] PPF_HD PPF_INLINE Gradient<Rank>
+  get_gradient(Types::Index field_index, DependencyType type) const
+  {
+    static_assert(Rank == Scalar || Rank == Vector, "Unknown field rank");
+    if constexpr (Rank == Vector)
+      {
+        Gradient<Rank> g;
+        for (unsigned int c = 0; c < dim; ++c)
+          g[c] = scalar_gradient(field_index + c, type);
+        return g;
+      }
+    else
+      return scalar_gradient(field_index, type);
+  }
+
+  /// Symmetric part of a vector field's gradient, (g + g^T) / 2 (reference
+  /// field_container.h get_symmetric_gradient).
+  template <TensorRank Rank, DependencyType type>
+  [[nodiscard]] PPF_HD PPF_INLINE Gradient<Rank>
+  get_symmetric_gradient(Types::Index field_index) const
+  {
+    static_assert(Rank == Vector, "get_symmetric_gradient needs a vector field");
+    static_assert(type != DST, "The destination of a solve cannot be read");
+    const Gradient<Rank> g = get_gradient<Rank>(field_index, type);
+    Gradient<Rank>       e;
+    for (unsigned int i = 0; i < dim; ++i)
+      for (unsigned int j = 0; j < dim; ++j)
+        e[i][j] = ScalarValue(number(0.5) * (g[i][j].data + g[j][i].data));
+    return e;
+  }
+
+  [[nodiscard]] PPF_HD PPF_INLINE ScalarValue
+  scalar_value(Types::Index field_index, DependencyType type) const
+  {
     const int slot = dependency_slot(type);
     read_value_mask[slot] |= 1u << field_index;
     if (cell_view)
@@ -70,23 +128,12 @@ public:
     return ScalarValue(value[field_index][slot]);
   }
 
-  template <TensorRank Rank, DependencyType type>
-  [[nodiscard]] PPF_HD PPF_INLINE Gradient<Rank>
-  get_gradient(Types::Index field_index) const
+  [[nodiscard]] PPF_HD PPF_INLINE Gradient<Scalar>
+  scalar_gradient(Types::Index field_index, DependencyType type) const
   {
-    static_assert(Rank == Scalar, "Only scalar fields are supported on the MI355X path");
-    static_assert(type != DST, "The destination of a solve cannot be read");
-    return get_gradient<Rank>(field_index, type);
-  }
-
-  template <TensorRank Rank>
-  [[nodiscard]] PPF_HD PPF_INLINE Gradient<Rank>
-  get_gradient(Types::Index field_index, DependencyType type) const
-  {
-    static_assert(Rank == Scalar, "Only scalar fields are supported on the MI355X path");
     const int slot = dependency_slot(type);
     read_gradient_mask[slot] |= 1u << field_index;
-    Gradient<Rank> g;
+    Gradient<Scalar> g;
     if (cell_view)
       {
         // Collocation: the gradient at node q of the cell is the 1-D derivative matrix
@@ -175,9 +222,16 @@ public:
       value_term[field_index] = static_cast<number>(val.data);
     else if constexpr (std::is_arithmetic_v<ValType>)
       value_term[field_index] = static_cast<number>(val);
+    else if constexpr (dealii::is_tensor<ValType>::value && ValType::rank == 1)
+      {
+        // Vector field: one value term per component field.
+        for (unsigned int c = 0; c < dim; ++c)
+          set_value_term(field_index + c, val[c]);
+        return;
+      }
     else
       static_assert(dependent_false<ValType>::value,
-                    "Only scalar value terms are supported on the MI355X path");
+                    "Only scalar and vector value terms are supported on the MI355X path");
     value_term_mask |= 1u << field_index;
   }
 
@@ -185,8 +239,24 @@ public:
   PPF_HD PPF_INLINE void
   set_gradient_term(Types::Index field_index, const GradType &val)
   {
-    static_assert(dealii::is_tensor<GradType>::value && GradType::rank == 1,
-                  "Only scalar-field gradient terms are supported on the MI355X path");
+    static_assert(dealii::is_tensor<GradType>::value && (GradType::rank == 1 ||
+                                                         GradType::rank == 2),
+                  "Gradient terms are rank 1 (scalar field) or rank 2 (vector field)");
+    if constexpr (GradType::rank == 2)
+      {
+        // Vector field: row c is the gradient term of component c.
+        for (unsigned int c = 0; c < dim; ++c)
+          set_gradient_term(field_index + c, val[c]);
+        return;
+      }
+    else
+      set_scalar_gradient_term(field_index, val);
+  }
+
+  template <typename GradType>
+  PPF_HD PPF_INLINE void
+  set_scalar_gradient_term(Types::Index field_index, const GradType &val)
+  {
     for (unsigned int d = 0; d < dim; ++d)
       {
         if constexpr (dealii::is_lane_value<typename GradType::value_type>::value)

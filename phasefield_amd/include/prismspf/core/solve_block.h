@@ -144,4 +144,72 @@ validate_solve_blocks(const std::vector<SolveBlock>      &solve_blocks,
   return solve_blocks;
 }
 
+/// Declared field and component of a stored (scalar) field.
+struct StoredComponent
+{
+  Types::Index declared  = 0;
+  unsigned int component = 0;
+  bool         vector    = false;
+};
+
+/// Vector fields (TensorRank::Vector, reference field_attributes.h:25-101) are stored as
+/// `dim` scalar component fields at consecutive indices: field k of rank Vector becomes
+/// the stored fields k .. k+dim-1, named <name>_x, <name>_y, <name>_z, each carrying the
+/// boundary conditions of its component (`u:x` in the parameter file).  Only one vector
+/// field, declared last, is supported, so that the declared index of every field is also
+/// the stored index of its first component (the index the equations use).  Solve blocks
+/// and dependency sets that name the vector field are widened to all its components.
+/// Returns the declared field and component of every stored field.
+inline std::vector<StoredComponent>
+expand_vector_fields(std::vector<FieldAttributes> &fields, std::vector<SolveBlock> &blocks,
+                     unsigned int dim)
+{
+  std::vector<StoredComponent> layout;
+  for (Types::Index f = 0; f < fields.size(); ++f)
+    if (fields[f].field_type == Vector)
+      {
+        if (f + 1 != fields.size())
+          throw std::invalid_argument("field " + fields[f].name +
+                                      ": a vector field must be the last declared field");
+      }
+    else if (fields[f].field_type != Scalar)
+      throw std::invalid_argument("field " + fields[f].name + ": rank-2 fields are not "
+                                  "supported");
+  for (Types::Index f = 0; f < fields.size(); ++f)
+    layout.push_back({f, 0u, false});
+  if (fields.empty() || fields.back().field_type != Vector)
+    return layout;
+  const Types::Index   v    = fields.size() - 1;
+  const FieldAttributes base = fields.back();
+  static const char   *axes = "xyz";
+  fields.pop_back();
+  layout.pop_back();
+  for (unsigned int c = 0; c < dim; ++c)
+    {
+      FieldAttributes fc = base;
+      fc.name            = base.name + "_" + axes[c];
+      fc.field_type      = Scalar;
+      fc.boundary_conditions.component_constraints.clear();
+      const auto it = base.boundary_conditions.component_constraints.find(c);
+      if (it != base.boundary_conditions.component_constraints.end())
+        fc.boundary_conditions.component_constraints[0] = it->second;
+      fields.push_back(fc);
+      layout.push_back({v, c, true});
+    }
+  for (SolveBlock &b : blocks)
+    {
+      if (b.field_indices.count(v))
+        for (unsigned int c = 1; c < dim; ++c)
+          b.field_indices.insert(v + c);
+      for (DependencyMap *deps : {&b.dependencies_rhs, &b.dependencies_lhs})
+        {
+          const auto it = deps->find(v);
+          if (it != deps->end())
+            for (unsigned int c = 1; c < dim; ++c)
+              (*deps)[v + c] = it->second;
+        }
+    }
+  return layout;
+}
+
 PRISMS_PF_END_NAMESPACE

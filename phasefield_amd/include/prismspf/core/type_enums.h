@@ -16,7 +16,7 @@ enum SolveType
   Newton
 };
 
-/// Rank of a field.  Only scalar fields run on the device path.
+/// Rank of a field.  A vector field is stored as dim scalar component fields.
 enum TensorRank
 {
   Undefined = static_cast<unsigned int>(-1),

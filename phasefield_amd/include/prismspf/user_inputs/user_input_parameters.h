@@ -257,6 +257,42 @@ public:
     return t;
   }
 
+  /// `(E, nu), isotropic elastic constants` as a Voigt stiffness matrix (reference
+  /// user_constants.h get_cij_tensor / get_cij_matrix, isotropic case):
+  /// lambda = nu E / ((1 + nu)(1 - 2 nu)) (plane strain in 2-D), shear modulus
+  /// mu = E / (2 (1 + nu)); C = lambda on the normal off-diagonals, lambda + 2 mu on the
+  /// normal diagonal, mu on the shear diagonal.  Other symmetries are not supported.
+  [[nodiscard]] dealii::Tensor<2, dim *(dim + 1) / 2>
+  get_elasticity_tensor(const std::string &name) const
+  {
+    const auto it = values.find(name);
+    if (it == values.end())
+      throw std::invalid_argument("Model constant " + name + " is not set");
+    std::string t = it->second.second;
+    std::transform(t.begin(), t.end(), t.begin(), ::tolower);
+    if (t.find("elastic constants") == std::string::npos)
+      throw std::invalid_argument("Model constant " + name + " is not elastic constants");
+    if (prm::trim(t.substr(0, t.find("elastic constants"))) != "isotropic")
+      throw std::invalid_argument("Model constant " + name +
+                                  ": only isotropic elastic constants are supported");
+    std::string v = it->second.first;
+    v.erase(std::remove(v.begin(), v.end(), '('), v.end());
+    v.erase(std::remove(v.begin(), v.end(), ')'), v.end());
+    const auto parts = prm::split(v, ',');
+    if (parts.size() != 2)
+      throw std::invalid_argument("Model constant " + name +
+                                  ": isotropic elastic constants are (E, nu)");
+    const double E = std::stod(parts[0]), nu = std::stod(parts[1]);
+    const double mu     = E / (2.0 * (1.0 + nu));
+    const double lambda = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    dealii::Tensor<2, dim *(dim + 1) / 2> C;
+    for (unsigned int a = 0; a < dim * (dim + 1) / 2; ++a)
+      for (unsigned int b = 0; b < dim * (dim + 1) / 2; ++b)
+        C[a][b] = a < dim && b < dim ? (a == b ? lambda + 2.0 * mu : lambda)
+                                     : (a == b ? mu : 0.0);
+    return C;
+  }
+
   void
   add(const std::string &name, const std::string &raw)
   {

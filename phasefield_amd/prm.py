@@ -231,6 +231,34 @@ def testbed(dim=2, refinement=3, size=4.0, steps=2, dt=1e-3, dirichlet_faces=(0,
     ]) + "\n"
 
 
+def mechanics(size=100.0, refinement=6, E=2.0, nu=0.3, tolerance=1e-10,
+              tolerance_type="RMSEPerField", preconditioner=None, max_iterations=10000,
+              conditions=("Dirichlet", "Dirichlet", "Natural", "Dirichlet", "Natural",
+                          "Dirichlet")):
+    """The linear elasticity application's parameters (defaults: the reference's
+    applications/mechanics/boundary_value_problem/parameters.prm)."""
+    lines = [
+        mesh_block(3, size, refinement, periodic=False),
+        _common(0, 0.0),
+        "subsection linear solver parameters: 0",
+        "  set solver_ids = 1",
+        f"  set max iterations = {max_iterations}",
+        f"  set tolerance type = {tolerance_type}",
+        f"  set tolerance value = {tolerance!r}",
+    ]
+    if preconditioner:
+        lines.append(f"  set preconditioner = {preconditioner}")
+    lines += [
+        "end",
+        "subsection boundary conditions: 0",
+        "  set variables = u:x, u:y, u:z",
+        f"  set conditions = {', '.join(conditions)}",
+        "end",
+        f"set Model constant stiffness = ({E!r},{nu!r}), isotropic elastic constants",
+    ]
+    return "\n".join(lines) + "\n"
+
+
 def config_prm(name: str, **kw) -> str:
     """Parameter text of a benchmark configuration (keyword overrides shrink it)."""
     if name == "allen_cahn_2d":

@@ -676,3 +676,42 @@ def test_newton_minres_matches_dense_newton_and_scipy(oracle_libs):
     assert its >= 3 and ns["converged"] and ns["iterations"] == its
     assert rel(n1[mesh.unique], u) < 1e-9
     assert abs(ns["linear_iterations"] - steps) <= its, (ns, steps)
+
+
+def test_mechanics_matches_assembled_elasticity(oracle_libs):
+    """The oracle's linear-elasticity solve (one vector field stored as three component
+    fields, symmetric-gradient stress with isotropic constants) equals the solution of
+    the brute-force assembled system K[(i,a),(j,b)] = sum_q w_q (lambda G_ia G_jb +
+    mu delta_ab G_i.G_j + mu G_ib G_ja) with the same Dirichlet values (u_x = -1 on
+    x = 0, zero on the other Dirichlet faces), to 1e-8."""
+    E, nu, size = 2.0, 0.3, 100.0
+    text = prm.mechanics(size=size, refinement=1, tolerance=1e-13,
+                         tolerance_type="AbsoluteResidual")
+    o = oracle("mechanics_3d_q2", text)
+    o.solve_increment()
+    faces = (0, 1, 3, 5)
+    mesh = bf.Mesh(3, 2, [2, 2, 2], [size] * 3, [False] * 3, dirichlet_faces=faces)
+    G, wq = bf.element_gradients(mesh)
+    lam = nu * E / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    n = mesh.ndofs
+    K = np.zeros((3 * n, 3 * n))
+    for cell in mesh.cells_iter():
+        dofs, _ = mesh.cell_dofs(cell)
+        for a in range(3):
+            for b in range(3):
+                Ke = (lam * np.einsum("q,qi,qj->ij", wq, G[:, :, a], G[:, :, b])
+                      + mu * np.einsum("q,qi,qj->ij", wq, G[:, :, b], G[:, :, a]))
+                if a == b:
+                    Ke += mu * np.einsum("q,qid,qjd->ij", wq, G, G)
+                K[np.ix_(a * n + dofs, b * n + dofs)] += Ke
+    x = mesh.coords()
+    fixed = np.concatenate([mesh.dirichlet] * 3)
+    g = np.zeros(3 * n)
+    g[:n][mesh.dirichlet & (x[:, 0] == 0)] = -1.0
+    u = g.copy()
+    free = ~fixed
+    u[free] = np.linalg.solve(K[np.ix_(free, free)], -K[np.ix_(free, fixed)] @ g[fixed])
+    got = np.concatenate([o.get_field(c, 0) for c in range(3)])
+    assert np.abs(u[:n]).max() > 0.1 and np.abs(u[n:]).max() > 1e-3
+    assert np.abs(got - u).max() <= 1e-8 * np.abs(u).max(), np.abs(got - u).max()
