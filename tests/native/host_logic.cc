@@ -129,5 +129,44 @@ end
   std::cout << "unterminated: "
             << error_of([&] { UserInputParameters<2>::from_text("subsection output\n"); })
             << "\n";
+
+  // Vector fields: component expansion and isotropic elastic constants.
+  {
+    std::vector<FieldAttributes> vf = {FieldAttributes("T"), FieldAttributes("u", Vector)};
+    BoundaryConditionSet         bcs;
+    bcs.component_constraints[1][0] = Condition::Dirichlet;
+    vf[1].boundary_conditions       = bcs;
+    SolveBlock vb;
+    vb.id               = 1;
+    vb.solve_type       = Linear;
+    vb.field_indices    = {1};
+    vb.dependencies_lhs = make_dependency_set(vf, {"grad(lhs(u))"});
+    std::vector<SolveBlock> vbs = {vb};
+    const auto              layout = expand_vector_fields(vf, vbs, 3);
+    std::cout << "vec_names:";
+    for (const auto &f : vf)
+      std::cout << " " << f.name;
+    std::cout << "\nvec_layout:";
+    for (const auto &l : layout)
+      std::cout << " " << l.declared << "." << l.component << (l.vector ? "v" : "s");
+    std::cout << "\nvec_block:";
+    for (auto f : vbs[0].field_indices)
+      std::cout << " " << f << ":" << int(vbs[0].dependencies_lhs.at(f).src_flag);
+    std::cout << "\nvec_bc: " << vf[1].boundary_conditions.component_constraints.count(0) << " "
+              << int(vf[2].boundary_conditions.component_constraints.at(0).at(0)) << "\n";
+    std::vector<FieldAttributes> bad = {FieldAttributes("u", Vector), FieldAttributes("T")};
+    std::vector<SolveBlock>      none;
+    std::cout << "vec_not_last: " << error_of([&] { expand_vector_fields(bad, none, 3); })
+              << "\n";
+    const auto q = UserInputParameters<3>::from_text(
+      "set Model constant stiffness = (2.0,0.25), isotropic elastic constants\n"
+      "set Model constant ortho = (1,2,3,4,5,6,7,8,9), orthotropic elastic constants\n");
+    const auto C = q.user_constants.get_elasticity_tensor("stiffness");
+    std::cout << "stiffness: " << C[0][0] << " " << C[0][1] << " " << C[3][3] << " "
+              << C[0][3] << "\n";
+    std::cout << "ortho: "
+              << error_of([&] { (void) q.user_constants.get_elasticity_tensor("ortho"); })
+              << "\n";
+  }
   return 0;
 }

@@ -88,3 +88,16 @@ def test_libraries_export_every_declared_symbol(product_libs):
         lib.ppf_app_name.restype = ctypes.c_char_p
         app = path.name[len("libppf_"):-len(".so")]
         assert lib.ppf_app_name().decode() == app
+
+
+def test_vector_field_expansion_and_elastic_constants(host_logic):
+    # A vector field u (declared last) becomes u_x, u_y, u_z; the block and its lhs
+    # dependency are widened; component 1's boundary conditions go to u_y.
+    assert host_logic["vec_names"] == "T u_x u_y u_z"
+    assert host_logic["vec_layout"] == "0.0s 1.0v 1.1v 1.2v"
+    assert host_logic["vec_block"] == "1:2 2:2 3:2"        # src gradients of every component
+    assert host_logic["vec_bc"] == "0 1"                  # u_x: none; u_y: Dirichlet on face 0
+    assert "must be the last declared field" in host_logic["vec_not_last"]
+    # E = 2, nu = 0.25: lambda = 0.8, mu = 0.8 -> C11 = 2.4, C12 = 0.8, C44 = 0.8.
+    assert host_logic["stiffness"] == "2.4 0.8 0.8 0"
+    assert "only isotropic" in host_logic["ortho"]
