@@ -51,7 +51,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="target duration of the CPU-baseline sample (0 disables it)")
     p.add_argument("--no-extra", action="store_true", help="skip every extra measurement")
-    p.add_argument("--extras", default="poisson,adjoint,newton,precipitate",
+    p.add_argument("--extras", default="poisson,adjoint,newton,precipitate,mechanics",
                    help="comma-separated extra measurements to run")
     p.add_argument("--kernel-reps", type=int, default=50)
     p.add_argument("--copy-calibration", action="store_true",
@@ -318,6 +318,39 @@ def precipitate_extra(torch):
     return res
 
 
+def mechanics_extra(torch):
+    """Linear elasticity (one vector field, three components) at the reference
+    application's parameters (applications/mechanics/boundary_value_problem/
+    parameters.prm: 64^3 Q2 cells on [0,100]^3, RMSEPerField 1e-10, CG without a
+    preconditioner), and with a Chebyshev preconditioner."""
+    from phasefield_amd import Context, prm
+    from phasefield_amd.capi import OP_LHS, SCALE_NONE
+
+    res = {"workload": "Linear elasticity 3D, Q2, 64^3 cells on [0,100]^3, fields u_x, u_y, "
+                       "u_z, E=2, nu=0.3, CG to RMSEPerField 1e-10"}
+    for label, pc in (("no_preconditioner", None), ("chebyshev", "Chebyshev")):
+        ctx = Context("mechanics_3d_q2", prm.mechanics(preconditioner=pc))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.run(0)
+        ctx.synchronize()
+        secs = time.perf_counter() - t0
+        st = ctx.last_solve_stats(1)
+        r = {"seconds_with_setup": secs, "iterations": st.iterations,
+             "converged": bool(st.converged), "ms_per_iteration": secs / max(1, st.iterations) * 1e3}
+        if pc is None:
+            n = 3 * ctx.info.n_nodes
+            src = torch.randn(n, dtype=torch.float64, device="cuda")
+            dst = torch.zeros_like(src)
+            t = time_kernel(ctx, torch, 1, OP_LHS, SCALE_NONE, src, dst, 20)
+            r["lhs_apply_us"] = t * 1e6
+            r["lhs_apply_dofs_per_s"] = n / t
+            del src, dst
+        res[label] = r
+        del ctx
+    return res
+
+
 def newton_extra(torch):
     """Implicit Allen-Cahn (Newton + CG per increment) at the reference application's
     parameters (applications/allen_cahn/implicit/parameters.prm: 256^2 cells on
@@ -480,7 +513,7 @@ def main():
         extra = poisson_extra(torch) if "poisson" in wanted else {}
         torch.cuda.empty_cache()
         for name, fn in (("adjoint", adjoint_extra), ("newton", newton_extra),
-                         ("precipitate", precipitate_extra)):
+                         ("precipitate", precipitate_extra), ("mechanics", mechanics_extra)):
             if name in wanted:
                 extra[name] = fn(torch)
                 torch.cuda.empty_cache()
