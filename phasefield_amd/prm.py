@@ -233,7 +233,7 @@ def testbed(dim=2, refinement=3, size=4.0, steps=2, dt=1e-3, dirichlet_faces=(0,
 
 def mechanics(size=100.0, refinement=6, E=2.0, nu=0.3, tolerance=1e-10,
               tolerance_type="RMSEPerField", preconditioner=None, max_iterations=10000,
-              conditions=("Dirichlet", "Dirichlet", "Natural", "Dirichlet", "Natural",
+              mg_depth=None, conditions=("Dirichlet", "Dirichlet", "Natural", "Dirichlet", "Natural",
                           "Dirichlet")):
     """The linear elasticity application's parameters (defaults: the reference's
     applications/mechanics/boundary_value_problem/parameters.prm)."""
@@ -248,6 +248,8 @@ def mechanics(size=100.0, refinement=6, E=2.0, nu=0.3, tolerance=1e-10,
     ]
     if preconditioner:
         lines.append(f"  set preconditioner = {preconditioner}")
+    if mg_depth is not None:
+        lines.append(f"  set mg depth = {mg_depth}")
     lines += [
         "end",
         "subsection boundary conditions: 0",

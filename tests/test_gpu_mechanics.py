@@ -2,8 +2,8 @@
 applications/mechanics/boundary_value_problem; one vector field u stored as the component
 fields u_x, u_y, u_z; lhs stress C eps(u) from get_symmetric_gradient) against the
 oracle: CG iterations within one, displacements within 1e-8 relative.  Cases: the
-reference's boundary conditions and RMSE tolerance without a preconditioner and with
-Chebyshev, and an absolute tolerance on a finer mesh.  The oracle's solve is pinned
+reference's boundary conditions and RMSE tolerance without a preconditioner, with
+Chebyshev and with GMG (two levels), and an absolute tolerance on a finer mesh.  The oracle's solve is pinned
 against a brute-force assembled elasticity system in tests/test_oracle.py.
 """
 
@@ -24,6 +24,7 @@ CASES = {
     "rmse_none": dict(refinement=2, tolerance=1e-10),
     "rmse_chebyshev": dict(refinement=2, tolerance=1e-10, preconditioner="Chebyshev"),
     "absolute_none": dict(refinement=3, tolerance=1e-9, tolerance_type="AbsoluteResidual"),
+    "rmse_gmg": dict(refinement=3, tolerance=1e-10, preconditioner="GMG", mg_depth=2),
 }
 
 
