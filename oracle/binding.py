@@ -66,6 +66,13 @@ class Oracle:
             ]
             lib.oracle_mg_level_ndofs.argtypes = [c_void_p, c_int, c_int, POINTER(c_longlong)]
             lib.oracle_mg_transfer.argtypes = [c_void_p, c_int, c_int, c_int, _dp, _dp]
+            lib.oracle_mg_inv_diag.argtypes = [c_void_p, c_int, c_int, _dp]
+            lib.oracle_mg_smooth.argtypes = [c_void_p, c_int, c_int, _dp, _dp, _dp, _dp]
+            lib.oracle_mg_estimate.argtypes = [c_void_p, c_int, c_int, _dp, _dp, _dp]
+            lib.oracle_mg_vcycle.argtypes = [c_void_p, c_int, _dp, _dp]
+            lib.oracle_mg_level_vmult.argtypes = [c_void_p, c_int, c_int, _dp, _dp]
+            lib.oracle_precondition.argtypes = [c_void_p, c_int, _dp, _dp]
+            lib.oracle_krylov_steps.argtypes = [c_void_p, c_int, _dp, _dp, c_uint]
             Oracle._libs[app] = lib
         self.lib = Oracle._libs[app]
         if threads:
@@ -196,3 +203,54 @@ class Oracle:
         out = np.empty(self.mg_level_ndofs(block_id, level - 1))
         self._check(self.lib.oracle_mg_transfer(self.h, block_id, level, 1, _ptr(f), _ptr(out)))
         return out
+
+    def mg_inv_diag(self, block_id, level):
+        """Inverse diagonal of V-cycle level `level` (coarsest first), as its smoother uses it."""
+        out = np.empty(self.mg_level_ndofs(block_id, level))
+        self._check(self.lib.oracle_mg_inv_diag(self.h, block_id, level, _ptr(out)))
+        return out
+
+    def mg_smooth(self, block_id, level, rhs):
+        """x = S rhs: one application of the level's Chebyshev smoother from zero;
+        returns (x, lmax, lmin)."""
+        r = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.empty_like(r)
+        lmax, lmin = c_double(), c_double()
+        self._check(self.lib.oracle_mg_smooth(self.h, block_id, level, _ptr(r), _ptr(x),
+                                              ctypes.byref(lmax), ctypes.byref(lmin)))
+        return x, lmax.value, lmin.value
+
+    def mg_estimate(self, block_id, level, x0):
+        """The smoother's eigenvalue estimate from start vector x0: (lmax, lmin)."""
+        v = np.ascontiguousarray(x0, dtype=np.float64)
+        lmax, lmin = c_double(), c_double()
+        self._check(self.lib.oracle_mg_estimate(self.h, block_id, level, _ptr(v),
+                                                ctypes.byref(lmax), ctypes.byref(lmin)))
+        return lmax.value, lmin.value
+
+    def mg_vcycle(self, block_id, defect):
+        """u = V(defect): one V-cycle on the finest level."""
+        d = np.ascontiguousarray(defect, dtype=np.float64)
+        u = np.empty_like(d)
+        self._check(self.lib.oracle_mg_vcycle(self.h, block_id, _ptr(d), _ptr(u)))
+        return u
+
+    def mg_level_vmult(self, block_id, level, src):
+        x = np.ascontiguousarray(src, dtype=np.float64)
+        y = np.empty_like(x)
+        self._check(self.lib.oracle_mg_level_vmult(self.h, block_id, level, _ptr(x), _ptr(y)))
+        return y
+
+    def precondition(self, block_id, g):
+        """h = P^{-1} g with the block's preconditioner, set up as for a solve."""
+        v = np.ascontiguousarray(g, dtype=np.float64)
+        out = np.empty_like(v)
+        self._check(self.lib.oracle_precondition(self.h, block_id, _ptr(v), _ptr(out)))
+        return out
+
+    def krylov_steps(self, block_id, x0, rhs, steps):
+        """Exactly `steps` steps of the block's GMRES / FGMRES from x0."""
+        x = np.array(x0, dtype=np.float64)
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        self._check(self.lib.oracle_krylov_steps(self.h, block_id, _ptr(x), _ptr(b), steps))
+        return x

@@ -785,7 +785,14 @@ namespace oracle
           v *= std::sqrt(vol);
         }
       if (t == prismspf::RMSEPerField || t == prismspf::IntegratedPerField)
-        v *= std::sqrt(double(blocks[b].field_indices.size()));
+        {
+          // Declared fields, not stored components: a vector field counts once,
+          // as in the reference's solve_block.field_indices.
+          std::set<prismspf::Types::Index> declared;
+          for (const auto f : blocks[b].field_indices)
+            declared.insert(layout[f].declared);
+          v *= std::sqrt(double(declared.size()));
+        }
       return v;
     }
 
@@ -1582,6 +1589,77 @@ namespace oracle
       return st;
     }
 
+    /// The block's preconditioner set up for a solve (reference linear_solver.h:348-379):
+    /// Chebyshev on the fine level (inverse diagonal re-evaluated every solve, reference
+    /// linear_solver.h:362-367; eigenvalues estimated at the preconditioner's first
+    /// application only) or the multigrid levels.
+    void
+    prepare_preconditioner(int b)
+    {
+      const auto &lp = blocks[b].linear_solver_parameters;
+      const int   sc = scaling_of(b);
+      if (lp.preconditioner == prismspf::Chebyshev)
+        {
+          fine_cheb.inv_diag = inverse_diagonal(b, top, cons, jxw, storage, sc);
+          auto it            = cheb_eigenvalues.find(b);
+          if (it == cheb_eigenvalues.end())
+            {
+              estimate(top_operator(b), fine_cheb, start_vector(top, int(block_fields(b).size())),
+                       lp.chebyshev_parameters);
+              cheb_eigenvalues[b] = {fine_cheb.lmax, fine_cheb.lmin};
+            }
+          else
+            {
+              fine_cheb.lmax = it->second.first;
+              fine_cheb.lmin = it->second.second;
+            }
+        }
+      else if (lp.preconditioner == prismspf::GMG)
+        setup_mg(b);
+    }
+
+    /// h = P^{-1} g with the preconditioner prepared by prepare_preconditioner.
+    void
+    precondition(int b, const std::vector<double> &g, std::vector<double> &h) const
+    {
+      const auto &lp = blocks[b].linear_solver_parameters;
+      if (lp.preconditioner == prismspf::None)
+        h = g;
+      else if (lp.preconditioner == prismspf::Chebyshev)
+        {
+          h.assign(g.size(), 0.0);
+          chebyshev(top_operator(b), fine_cheb, lp.chebyshev_parameters.degree, h, g, true);
+        }
+      else
+        {
+          h.assign(g.size(), 0.0);
+          v_cycle(b, int(mg.size()) - 1, h, g);
+        }
+    }
+
+    /// The lhs operator of block b on the finest level, scaled for the linear solver.
+    Op
+    top_operator(int b) const
+    {
+      const int sc = scaling_of(b);
+      return [this, b, sc](const std::vector<double> &x, std::vector<double> &y) {
+        y.assign(x.size(), 0.0);
+        apply(b, true, top, cons, jxw, storage, x.data(), y.data(), sc, false);
+      };
+    }
+
+    /// The lhs operator of block b on multigrid level li (homogeneous level constraints).
+    Op
+    level_operator(int b, int li) const
+    {
+      const int sc = scaling_of(b);
+      return [this, b, li, sc](const std::vector<double> &x, std::vector<double> &y) {
+        const auto &L = mg[li].lvl;
+        y.assign(x.size(), 0.0);
+        apply(b, true, L.mesh, L.cons, L.jxw, L.fields, x.data(), y.data(), sc, false);
+      };
+    }
+
     /// do_linear_solve (reference linear_solver.h:348-402): the block's preconditioner,
     /// set up for this solve, and deal.II SolverCG's recurrences from the initial guess x
     /// (absolute residual tolerance = tolerance value * normalization).
@@ -1596,38 +1674,9 @@ namespace oracle
         y.assign(x.size(), 0.0);
         apply_top(b, true, x.data(), y.data(), sc, false);
       };
-      if (lp.preconditioner == prismspf::Chebyshev)
-        {
-          // Diagonal re-evaluated every solve (reference linear_solver.h:362-367);
-          // eigenvalues estimated at the preconditioner's first application only.
-          fine_cheb.inv_diag = inverse_diagonal(b, top, cons, jxw, storage, sc);
-          auto it = cheb_eigenvalues.find(b);
-          if (it == cheb_eigenvalues.end())
-            {
-              estimate(A, fine_cheb, start_vector(top, int(bf.size())), lp.chebyshev_parameters);
-              cheb_eigenvalues[b] = {fine_cheb.lmax, fine_cheb.lmin};
-            }
-          else
-            {
-              fine_cheb.lmax = it->second.first;
-              fine_cheb.lmin = it->second.second;
-            }
-        }
-      else if (lp.preconditioner == prismspf::GMG)
-        setup_mg(b);
+      prepare_preconditioner(b);
       auto P = [&](const std::vector<double> &g, std::vector<double> &h) {
-        if (lp.preconditioner == prismspf::None)
-          h = g;
-        else if (lp.preconditioner == prismspf::Chebyshev)
-          {
-            h.assign(g.size(), 0.0);
-            chebyshev(A, fine_cheb, lp.chebyshev_parameters.degree, h, g, true);
-          }
-        else
-          {
-            h.assign(g.size(), 0.0);
-            v_cycle(b, int(mg.size()) - 1, h, g);
-          }
+        precondition(b, g, h);
       };
 
       Stats st;

@@ -336,4 +336,130 @@ oracle_mg_transfer(void *h, int block_id, int level, int direction, const double
   });
 }
 
+/// The inverse diagonal of multigrid level `level` of block `block_id` (full level
+/// lattice, zero diagonal entries replaced by one), as the level's smoother uses it.
+int
+oracle_mg_inv_diag(void *h, int block_id, int level, double *out)
+{
+  return oracle::guard(h, [&](auto &m) {
+    const int b = m.block_index(block_id);
+    m.setup_mg(b);
+    if (level < 0 || level >= int(m.mg.size()))
+      throw std::invalid_argument("level out of range");
+    const auto &d = m.mg[level].sm.inv_diag;
+    std::copy(d.begin(), d.end(), out);
+  });
+}
+
+/// One application of level `level`'s Chebyshev smoother (the block's degree, its
+/// eigenvalue bounds) from a zero start: x = S rhs.  Returns the bounds it used.
+int
+oracle_mg_smooth(void *h, int block_id, int level, const double *rhs, double *x, double *lmax,
+                 double *lmin)
+{
+  return oracle::guard(h, [&](auto &m) {
+    const int b = m.block_index(block_id);
+    m.setup_mg(b);
+    if (level < 0 || level >= int(m.mg.size()))
+      throw std::invalid_argument("level out of range");
+    const std::size_t   n = m.mg[level].sm.inv_diag.size();
+    std::vector<double> r(rhs, rhs + n), u(n, 0.0);
+    m.chebyshev(m.level_operator(b, level), m.mg[level].sm,
+                m.blocks[b].linear_solver_parameters.chebyshev_parameters.degree, u, r, true);
+    std::copy(u.begin(), u.end(), x);
+    *lmax = m.mg[level].sm.lmax;
+    *lmin = m.mg[level].sm.lmin;
+  });
+}
+
+/// The smoother's eigenvalue estimate on level `level` from the start vector x0:
+/// lmax (safety factor included) and lmin = lmax / smoothing range.
+int
+oracle_mg_estimate(void *h, int block_id, int level, const double *x0, double *lmax,
+                   double *lmin)
+{
+  return oracle::guard(h, [&](auto &m) {
+    const int b = m.block_index(block_id);
+    m.setup_mg(b);
+    if (level < 0 || level >= int(m.mg.size()))
+      throw std::invalid_argument("level out of range");
+    auto                sm = m.mg[level].sm;
+    std::vector<double> start(x0, x0 + sm.inv_diag.size());
+    m.estimate(m.level_operator(b, level), sm, start,
+               m.blocks[b].linear_solver_parameters.chebyshev_parameters);
+    *lmax = sm.lmax;
+    *lmin = sm.lmin;
+  });
+}
+
+/// One V-cycle of block `block_id` (the GMG preconditioner): u = V(defect) on the
+/// finest level.
+int
+oracle_mg_vcycle(void *h, int block_id, const double *defect, double *u)
+{
+  return oracle::guard(h, [&](auto &m) {
+    const int b = m.block_index(block_id);
+    m.setup_mg(b);
+    const std::size_t   n = m.mg.back().sm.inv_diag.size();
+    std::vector<double> d(defect, defect + n), out(n, 0.0);
+    m.v_cycle(b, int(m.mg.size()) - 1, out, d);
+    std::copy(out.begin(), out.end(), u);
+  });
+}
+
+/// Level operator of block `block_id` on level `level`: dst = A_level src.
+int
+oracle_mg_level_vmult(void *h, int block_id, int level, const double *src, double *dst)
+{
+  return oracle::guard(h, [&](auto &m) {
+    const int b = m.block_index(block_id);
+    m.setup_mg(b);
+    if (level < 0 || level >= int(m.mg.size()))
+      throw std::invalid_argument("level out of range");
+    const std::size_t   n = m.mg[level].sm.inv_diag.size();
+    std::vector<double> x(src, src + n), y;
+    m.level_operator(b, level)(x, y);
+    std::copy(y.begin(), y.end(), dst);
+  });
+}
+
+/// h = P^{-1} g with block `block_id`'s preconditioner, set up as for a solve.
+int
+oracle_precondition(void *h, int block_id, const double *g, double *out)
+{
+  return oracle::guard(h, [&](auto &m) {
+    const int b = m.block_index(block_id);
+    m.prepare_preconditioner(b);
+    const std::size_t   n = m.block_fields(b).size() * std::size_t(m.top.ndofs);
+    std::vector<double> gv(g, g + n), hv;
+    m.precondition(b, gv, hv);
+    std::copy(hv.begin(), hv.end(), out);
+  });
+}
+
+/// Exactly `steps` steps of block `block_id`'s Krylov solver (its solver type,
+/// preconditioner and GMRES basis size) on lhs x = rhs from x (tolerance 0, so the
+/// iteration stops at the step limit); x is updated in place.
+int
+oracle_krylov_steps(void *h, int block_id, double *x, const double *rhs, unsigned steps)
+{
+  return oracle::guard(h, [&](auto &m) {
+    const int b = m.block_index(block_id);
+    m.prepare_preconditioner(b);
+    const auto         &lp = m.blocks[b].linear_solver_parameters;
+    const std::size_t   n  = m.block_fields(b).size() * std::size_t(m.top.ndofs);
+    std::vector<double> xv(x, x + n), bv(rhs, rhs + n);
+    auto                A = m.top_operator(b);
+    auto P = [&](const std::vector<double> &g, std::vector<double> &hv) {
+      m.precondition(b, g, hv);
+    };
+    if (lp.solver_type == "gmres" || lp.solver_type == "fgmres")
+      m.gmres(A, P, xv, bv, 0.0, steps, int(lp.gmres_max_basis_size),
+              lp.gmres_right_preconditioning, lp.solver_type == "fgmres");
+    else
+      throw std::invalid_argument("oracle_krylov_steps: gmres or fgmres only");
+    std::copy(xv.begin(), xv.end(), x);
+  });
+}
+
 } // extern "C"

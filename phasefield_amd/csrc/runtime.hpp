@@ -38,6 +38,7 @@
 #include <sstream>
 #include <string>
 #include <utility>
+#include <set>
 #include <vector>
 
 namespace ppf
@@ -484,15 +485,27 @@ namespace ppf
           lower[d] = fine.lower[d];
           step[d]  = fine.h[d] / degree;
         }
+      // One output array per declared field: the dim component fields of a vector field
+      // (stored as <name>_x, _y, _z) form one 3-component array named <name>.
       std::vector<std::string>         names;
       std::vector<std::vector<double>> values;
+      std::vector<int>                 groups;
       for (std::size_t f = 0; f < fields.size(); ++f)
         {
-          names.push_back(fields[f].name);
+          if (!layout[f].vector)
+            {
+              names.push_back(fields[f].name);
+              groups.push_back(1);
+            }
+          else if (layout[f].component == 0)
+            {
+              names.push_back(fields[f].name.substr(0, fields[f].name.size() - 2));
+              groups.push_back(dim);
+            }
           values.emplace_back(full_size(fine));
           get_field(int(f), 0, values.back().data(), int64_t(values.back().size()));
         }
-      write_vtu(path.string(), dim, n, lower, step, names, values, timer.get_time(),
+      write_vtu(path.string(), dim, n, lower, step, names, values, groups, timer.get_time(),
                 timer.get_increment());
     }
 
@@ -533,6 +546,13 @@ namespace ppf
     {
       hipLaunchKernelGGL(axpby_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, stream, y, a,
                          x, b, n);
+    }
+
+    template <typename T>
+    void
+    scale(T *y, T b, int64_t n)
+    {
+      hipLaunchKernelGGL(scale_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, stream, y, b, n);
     }
 
     /// Deterministic dot product; the result is copied to the host.
@@ -1300,7 +1320,14 @@ namespace ppf
           v *= std::sqrt(vol);
         }
       if (type == prismspf::RMSEPerField || type == prismspf::IntegratedPerField)
-        v *= std::sqrt(double(blocks[b].field_indices.size()));
+        {
+          // Declared fields, not stored components: a vector field counts once,
+          // as in the reference's solve_block.field_indices.
+          std::set<prismspf::Types::Index> declared;
+          for (const auto f : blocks[b].field_indices)
+            declared.insert(layout[f].declared);
+          v *= std::sqrt(double(declared.size()));
+        }
       return v;
     }
 
@@ -1554,7 +1581,7 @@ namespace ppf
               copy(w, v, n);
               axpby(w, -oldeps, w1, 1.0, n);
               axpby(w, -delta, w2, 1.0, n);
-              axpby(w, 0.0, w, 1.0 / gamma, n);
+              scale(w, 1.0 / gamma, n);
               axpby(x, phi, w, 1.0, n);
               state = control_check(phibar, tol, step, max_steps);
               if (beta == 0.0 && state == 0)
@@ -1681,7 +1708,7 @@ namespace ppf
           state = control_check(rho, tol, its, max_steps);
           if (state != 0)
             break;
-          axpby(v0, 0.0, v0, 1.0 / rho, n);
+          scale(v0, 1.0 / rho, n);
           std::fill(g.begin(), g.end(), 0.0);
           g[0]  = rho;
           int j = 0;

@@ -71,6 +71,16 @@ namespace ppf
       y[i] = b == T(0) ? a * x[i] : a * x[i] + b * y[i];
   }
 
+  /// y = b*y (in place; one array, so no aliasing of restrict-qualified operands)
+  template <typename T>
+  __global__ void
+  scale_kernel(T *__restrict__ y, T b, int64_t n)
+  {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+      y[i] = b * y[i];
+  }
+
   /// z = x - y
   template <typename T>
   __global__ void

@@ -67,17 +67,31 @@ namespace ppf
   } // namespace vtu
 
   /// Writes `path`.  n[d]: nodes per direction of the full node set (n[2] = 1 in 2-D);
-  /// lower/step: coordinates of node 0 and the node spacing; values[f]: field f over the
-  /// full node set.
+  /// lower/step: coordinates of node 0 and the node spacing; values[f]: stored field f
+  /// over the full node set.  groups[g] consecutive stored fields form output array g,
+  /// named names[g]: a scalar field (1) is written as one component; the components of a
+  /// vector field (dim) are written as one 3-component array, the z component zero in
+  /// 2-D, as deal.II's DataOut writes a vector-valued field
+  /// (component_is_part_of_vector, reference solution_output.h:80-84).
   inline void
   write_vtu(const std::string &path, int dim, const int64_t (&n)[3], const double (&lower)[3],
             const double (&step)[3], const std::vector<std::string> &names,
-            const std::vector<std::vector<double>> &values, double time, unsigned int cycle)
+            const std::vector<std::vector<double>> &values, const std::vector<int> &groups,
+            double time, unsigned int cycle)
   {
     const int64_t n_points = n[0] * n[1] * n[2];
     for (const auto &v : values)
       if (int64_t(v.size()) != n_points)
         throw std::invalid_argument("write_vtu: field size does not match the node set");
+    std::size_t n_grouped = 0;
+    for (const int g : groups)
+      {
+        if (g != 1 && g != dim)
+          throw std::invalid_argument("write_vtu: an output array has 1 or dim components");
+        n_grouped += std::size_t(g);
+      }
+    if (names.size() != groups.size() || n_grouped != values.size())
+      throw std::invalid_argument("write_vtu: names, groups and fields do not match");
     const int64_t c0 = n[0] - 1, c1 = n[1] - 1, c2 = dim == 3 ? n[2] - 1 : 1;
     const int64_t n_cells = c0 * c1 * c2;
 
@@ -135,10 +149,29 @@ namespace ppf
     vtu::data_array(os, "Int64", "offsets", 1, offsets);
     vtu::data_array(os, "UInt8", "types", 1, types);
     os << "      </Cells>\n      <PointData>\n";
-    for (std::size_t f = 0; f < values.size(); ++f)
-      vtu::data_array(os, "Float64", names[f], 1, values[f]);
+    for (std::size_t g = 0, f = 0; g < groups.size(); f += std::size_t(groups[g]), ++g)
+      if (groups[g] == 1)
+        vtu::data_array(os, "Float64", names[g], 1, values[f]);
+      else
+        {
+          std::vector<double> vec(3 * n_points, 0.0);
+          for (int c = 0; c < groups[g]; ++c)
+            for (int64_t p = 0; p < n_points; ++p)
+              vec[3 * p + c] = values[f + std::size_t(c)][p];
+          vtu::data_array(os, "Float64", names[g], 3, vec);
+        }
     os << "      </PointData>\n    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n";
     if (!os)
       throw std::runtime_error("writing output file " + path + " failed");
+  }
+
+  /// Every stored field written as a scalar array.
+  inline void
+  write_vtu(const std::string &path, int dim, const int64_t (&n)[3], const double (&lower)[3],
+            const double (&step)[3], const std::vector<std::string> &names,
+            const std::vector<std::vector<double>> &values, double time, unsigned int cycle)
+  {
+    write_vtu(path, dim, n, lower, step, names, values, std::vector<int>(values.size(), 1), time,
+              cycle);
   }
 } // namespace ppf

@@ -185,6 +185,16 @@ namespace dealii
         values[d] = Number(0.0);
     }
 
+    /// Conversion from another number type (as deal.II's converting constructor; e.g.
+    /// a double stiffness constant held by an equation instantiated in float).
+    template <typename Other, typename = std::enable_if_t<!std::is_same_v<Other, Number>>>
+    PPF_HD constexpr Tensor(const Tensor<1, dim, Other> &o)
+      : values {}
+    {
+      for (int d = 0; d < dim; ++d)
+        values[d] = Number(o[d]);
+    }
+
     PPF_HD Number &
     operator[](unsigned int d)
     {
@@ -257,6 +267,14 @@ namespace dealii
     using value_type                                       = Number;
 
     PPF_HD constexpr Tensor() = default;
+
+    /// Conversion from another number type (see the rank-1 tensor).
+    template <typename Other, typename = std::enable_if_t<!std::is_same_v<Other, Number>>>
+    PPF_HD constexpr Tensor(const Tensor<2, dim, Other> &o)
+    {
+      for (int d = 0; d < dim; ++d)
+        values[d] = Tensor<1, dim, Number>(o[d]);
+    }
 
     PPF_HD Tensor<1, dim, Number> &
     operator[](unsigned int d)

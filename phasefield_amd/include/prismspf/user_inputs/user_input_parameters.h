@@ -215,10 +215,17 @@ struct OutputParameters
   }
 };
 
+/// Random-number engine of the applications' initial conditions (reference
+/// miscellaneous_parameters.h:22).
+using RNGEngine = std::mt19937;
+
+/// `random seed` and the engine seeded from it (reference miscellaneous_parameters.h:
+/// 58-65, set_random_seed in miscellaneous_parameters.cc:44-48; one process, so the seed
+/// is not offset by an MPI rank).
 struct MiscellaneousParameters
 {
-  unsigned int         random_seed = 2025;
-  mutable std::mt19937 rng {2025};
+  unsigned int      random_seed = 2025;
+  mutable RNGEngine rng {2025};
 };
 
 /// `Model constant` values (reference user_constants.h).
@@ -257,11 +264,13 @@ public:
     return t;
   }
 
-  /// `(E, nu), isotropic elastic constants` as a Voigt stiffness matrix (reference
-  /// user_constants.h get_cij_tensor / get_cij_matrix, isotropic case):
-  /// lambda = nu E / ((1 + nu)(1 - 2 nu)) (plane strain in 2-D), shear modulus
-  /// mu = E / (2 (1 + nu)); C = lambda on the normal off-diagonals, lambda + 2 mu on the
-  /// normal diagonal, mu on the shear diagonal.  Other symmetries are not supported.
+  /// `(E, nu), isotropic elastic constants [planestress | planestrain]` as a Voigt
+  /// stiffness matrix (reference user_constants.h construct_user_constant :432-480 and
+  /// get_cij_matrix :581-700, isotropic case): shear modulus mu = E / (2 (1 + nu));
+  /// lambda = nu E / ((1 + nu)(1 - 2 nu)) in 3-D and for 2-D plane strain (the 2-D
+  /// default), nu E / (1 - nu^2) for 2-D plane stress; C = lambda on the normal
+  /// off-diagonals, lambda + 2 mu on the normal diagonal, mu on the shear diagonal.  In
+  /// 1-D, C = E.  Other symmetries are not supported.
   [[nodiscard]] dealii::Tensor<2, dim *(dim + 1) / 2>
   get_elasticity_tensor(const std::string &name) const
   {
@@ -275,6 +284,18 @@ public:
     if (prm::trim(t.substr(0, t.find("elastic constants"))) != "isotropic")
       throw std::invalid_argument("Model constant " + name +
                                   ": only isotropic elastic constants are supported");
+    // Optional 2-D stress state after "elastic constants" (plane strain by default).
+    const std::string state =
+      prm::trim(t.substr(t.find("elastic constants") + std::string("elastic constants").size()));
+    bool plane_stress = false;
+    if (dim == 2)
+      {
+        if (state == "planestress")
+          plane_stress = true;
+        else if (!state.empty() && state != "planestrain")
+          throw std::invalid_argument(
+            "Model constant " + name + ": Wrong 2D stress state. Use planestress or planestrain.");
+      }
     std::string v = it->second.first;
     v.erase(std::remove(v.begin(), v.end(), '('), v.end());
     v.erase(std::remove(v.begin(), v.end(), ')'), v.end());
@@ -284,8 +305,14 @@ public:
                                   ": isotropic elastic constants are (E, nu)");
     const double E = std::stod(parts[0]), nu = std::stod(parts[1]);
     const double mu     = E / (2.0 * (1.0 + nu));
-    const double lambda = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double lambda = plane_stress ? nu * E / (1.0 - nu * nu)
+                                       : nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
     dealii::Tensor<2, dim *(dim + 1) / 2> C;
+    if (dim == 1)
+      {
+        C[0][0] = E;
+        return C;
+      }
     for (unsigned int a = 0; a < dim * (dim + 1) / 2; ++a)
       for (unsigned int b = 0; b < dim * (dim + 1) / 2; ++b)
         C[a][b] = a < dim && b < dim ? (a == b ? lambda + 2.0 * mu : lambda)

@@ -167,6 +167,23 @@ end
     std::cout << "ortho: "
               << error_of([&] { (void) q.user_constants.get_elasticity_tensor("ortho"); })
               << "\n";
+    const auto q2 = UserInputParameters<2>::from_text(
+      "set Model constant s0 = (2.0,0.25), isotropic elastic constants\n"
+      "set Model constant s1 = (2.0,0.25), isotropic elastic constants planestrain\n"
+      "set Model constant s2 = (2.0,0.25), isotropic elastic constants planestress\n"
+      "set Model constant s3 = (2.0,0.25), isotropic elastic constants planeflat\n");
+    for (const char *nm : {"s0", "s1", "s2"})
+      {
+        const auto C2 = q2.user_constants.get_elasticity_tensor(nm);
+        std::cout << "stiffness2d_" << nm << ": " << C2[0][0] << " " << C2[0][1] << " "
+                  << C2[2][2] << "\n";
+      }
+    std::cout << "stiffness2d_bad: "
+              << error_of([&] { (void) q2.user_constants.get_elasticity_tensor("s3"); })
+              << "\n";
+    const auto q1 = UserInputParameters<1>::from_text(
+      "set Model constant s = (2.0,0.25), isotropic elastic constants\n");
+    std::cout << "stiffness1d: " << q1.user_constants.get_elasticity_tensor("s")[0][0] << "\n";
   }
   return 0;
 }

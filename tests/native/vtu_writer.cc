@@ -28,6 +28,20 @@ main(int argc, char **argv)
             }
       ppf::write_vtu(dir + "/lattice_" + std::to_string(dim) + "d.vtu", dim, n, lower, step,
                      {"f", "g"}, values, 0.125 * dim, unsigned(7 * dim));
+      // A scalar field and a vector field u (components u_x = x, u_y = y, u_z = z).
+      std::vector<std::vector<double>> vec(1 + dim, std::vector<double>(n[0] * n[1] * n[2]));
+      for (int64_t k = 0, p = 0; k < n[2]; ++k)
+        for (int64_t j = 0; j < n[1]; ++j)
+          for (int64_t i = 0; i < n[0]; ++i, ++p)
+            {
+              vec[0][p] = double(p);
+              vec[1][p] = lower[0] + i * step[0];
+              vec[2][p] = lower[1] + j * step[1];
+              if (dim == 3)
+                vec[3][p] = lower[2] + k * step[2];
+            }
+      ppf::write_vtu(dir + "/vector_" + std::to_string(dim) + "d.vtu", dim, n, lower, step,
+                     {"g", "u"}, vec, {1, dim}, 1.0, 3u);
     }
   std::puts("ok");
   return 0;

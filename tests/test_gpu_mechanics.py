@@ -40,6 +40,9 @@ def test_mechanics_matches_oracle(device, name):
     print(name, "gpu its", st.iterations, "oracle its", ost["iterations"], flush=True)
     assert st.converged and ost["converged"]
     assert abs(st.iterations - ost["iterations"]) <= 1
+    # same normalization of the tolerance (one declared field, reference
+    # linear_solver.h:414-427)
+    assert st.tolerance == pytest.approx(ost["tolerance"], rel=1e-14)
     for c in range(3):
         g, o = ctx.get_field(c, AGE_LATEST).ravel(), orc.get_field(c, AGE_LATEST)
         assert np.abs(o).max() > 0.0 or c > 0

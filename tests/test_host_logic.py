@@ -101,3 +101,12 @@ def test_vector_field_expansion_and_elastic_constants(host_logic):
     # E = 2, nu = 0.25: lambda = 0.8, mu = 0.8 -> C11 = 2.4, C12 = 0.8, C44 = 0.8.
     assert host_logic["stiffness"] == "2.4 0.8 0.8 0"
     assert "only isotropic" in host_logic["ortho"]
+    # 2-D (reference get_cij_matrix): plane strain by default and when named, lambda =
+    # nu E/((1+nu)(1-2nu)) = 0.8; plane stress lambda = nu E/(1-nu^2) = 0.5/0.9375.
+    assert host_logic["stiffness2d_s0"] == "2.4 0.8 0.8"
+    assert host_logic["stiffness2d_s1"] == "2.4 0.8 0.8"
+    lam = 0.5 / 0.9375
+    c = [float(v) for v in host_logic["stiffness2d_s2"].split()]
+    assert c == pytest.approx([lam + 1.6, lam, 0.8], rel=1e-5)
+    assert "Wrong 2D stress state" in host_logic["stiffness2d_bad"]
+    assert host_logic["stiffness1d"] == "2"                 # 1-D: C = E

@@ -715,3 +715,34 @@ def test_mechanics_matches_assembled_elasticity(oracle_libs):
     got = np.concatenate([o.get_field(c, 0) for c in range(3)])
     assert np.abs(u[:n]).max() > 0.1 and np.abs(u[n:]).max() > 1e-3
     assert np.abs(got - u).max() <= 1e-8 * np.abs(u).max(), np.abs(got - u).max()
+
+
+@pytest.mark.parametrize("ttype", ["RMSEPerField", "IntegratedPerField", "RMSETotal",
+                                   "AbsoluteResidual"])
+def test_tolerance_normalization_counts_declared_fields(oracle_libs, ttype):
+    """normalization_value (reference linear_solver.h:414-427): sqrt(volume) for the RMSE
+    types times sqrt(number of the block's declared fields) for the PerField types.  The
+    elasticity block holds ONE declared field, the vector u (stored as three component
+    fields), so the factor is sqrt(1), not sqrt(3).  The absolute tolerance the solver
+    reports is tolerance value x that normalization."""
+    size, tol = 100.0, 1e-6
+    o = oracle("mechanics_3d_q2", prm.mechanics(size=size, refinement=1, tolerance=tol,
+                                                tolerance_type=ttype))
+    o.run(0)
+    got = o.linear_solve(1)["tolerance"]
+    vol = size ** 3
+    want = tol * {"RMSEPerField": np.sqrt(vol), "IntegratedPerField": 1.0,
+                  "RMSETotal": np.sqrt(vol), "AbsoluteResidual": 1.0}[ttype]
+    assert got == pytest.approx(want, rel=1e-14), (ttype, got, want)
+
+
+def test_tolerance_normalization_two_scalar_fields(oracle_libs):
+    """A Newton block of two scalar fields (implicit Cahn-Hilliard: c and mu) counts two
+    declared fields: tolerance = tolerance value x sqrt(volume) x sqrt(2) (RMSEPerField)."""
+    size = 100.0
+    text = prm.cahn_hilliard_implicit(size=size, refinement=2, steps=1, outputs=0,
+                                      newton_tolerance=1e-6)
+    o = oracle("cahn_hilliard_implicit_2d_q2", text)
+    o.run(1)
+    got = o.last_newton_stats(0)["tolerance"]
+    assert got == pytest.approx(1e-6 * np.sqrt(size ** 2) * np.sqrt(2.0), rel=1e-14)

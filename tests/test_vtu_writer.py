@@ -71,3 +71,16 @@ def test_vtu_round_trip(written, dim, n):
         unit += [(0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
     assert np.array_equal(d, np.broadcast_to(np.array(unit), d.shape))
     assert arrs["TIME"][0] == 0.125 * dim and arrs["CYCLE"][0] == 7 * dim
+
+
+@pytest.mark.parametrize("dim,n", [(2, (4, 3, 1)), (3, (3, 2, 2))])
+def test_vtu_vector_field_is_one_array(written, dim, n):
+    """A vector field's component fields are written as ONE 3-component array named
+    after the declared field (DataOut's component_is_part_of_vector, reference
+    solution_output.h:80-84), the z component zero in 2-D."""
+    arrs, n_points, _ = read_vtu(written / f"vector_{dim}d.vtu")
+    assert "u" in arrs and "u_x" not in arrs
+    u = arrs["u"]
+    assert u.shape == (n_points, 3)
+    np.testing.assert_array_equal(u, arrs["Points"])  # u = (x, y, z), z = 0 in 2-D
+    np.testing.assert_array_equal(arrs["g"], np.arange(n_points, dtype=np.float64))
