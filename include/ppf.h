@@ -154,6 +154,21 @@ int ppf_set_time(ppf_ctx *ctx, unsigned int increment, double time);
 int ppf_apply_operator(ppf_ctx *ctx, int block_id, int which, int scaling, int precision,
                        const void *d_src, void *d_dst);
 
+/* ppf_apply_operator without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results and status
+ * codes as ppf_apply_operator; the scratch the operator path needs (the generic cell
+ * path's per-cell results, the zero operand of an operator without gradient terms,
+ * FP32 copies of the fields an FP32 application reads) comes from a static device array
+ * of PPF_NOALLOC_SCRATCH_BYTES per GPU and application library, held until the process
+ * ends.  A launch that needs more fails with PPF_ERR_INVALID and a message naming the
+ * size (e.g. the generic path needs fields x (degree+1)^dim x 8 B per cell in FP64).
+ * Every call on a GPU shares that array: calls that overlap on one GPU (other threads,
+ * other streams) race.  The context itself (ppf_create) still allocates as usual. */
+#define PPF_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int ppf_apply_operator_noalloc(ppf_ctx *ctx, int block_id, int which, int scaling,
+                               int precision, const void *d_src, void *d_dst);
+
 /* MFOperator::compute_diagonal + eval_matrix_diagonal (src/solvers/mf_operator.cc:82-255):
  * inverse of the lhs diagonal with zero entries replaced by one, on a device array. */
 int ppf_inverse_diagonal(ppf_ctx *ctx, int block_id, int scaling, int precision, void *d_dst);

@@ -41,6 +41,7 @@ EXPORTED_SYMBOLS = (
     "ppf_rotate_old",
     "ppf_set_time",
     "ppf_apply_operator",
+    "ppf_apply_operator_noalloc",
     "ppf_inverse_diagonal",
     "ppf_linear_solve",
     "ppf_last_solve_stats",
@@ -143,6 +144,7 @@ class Library:
         L.ppf_rotate_old.argtypes = [vp, c_int]
         L.ppf_set_time.argtypes = [vp, c_uint, c_double]
         L.ppf_apply_operator.argtypes = [vp, c_int, c_int, c_int, c_int, vp, vp]
+        L.ppf_apply_operator_noalloc.argtypes = [vp, c_int, c_int, c_int, c_int, vp, vp]
         L.ppf_inverse_diagonal.argtypes = [vp, c_int, c_int, c_int, vp]
         L.ppf_linear_solve.argtypes = [vp, c_int, POINTER(SolveStats)]
         L.ppf_last_solve_stats.argtypes = [vp, c_int, POINTER(SolveStats)]
@@ -277,6 +279,14 @@ class Context:
     def apply_operator(self, block_id, which, scaling, precision, d_src, d_dst) -> None:
         self._check(
             self.L.ppf_apply_operator(
+                self.handle, block_id, which, scaling, precision,
+                c_void_p(d_src or 0), c_void_p(d_dst),
+            )
+        )
+
+    def apply_operator_noalloc(self, block_id, which, scaling, precision, d_src, d_dst) -> None:
+        self._check(
+            self.L.ppf_apply_operator_noalloc(
                 self.handle, block_id, which, scaling, precision,
                 c_void_p(d_src or 0), c_void_p(d_dst),
             )

@@ -19,11 +19,46 @@
 
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
 namespace ppf
 {
+  /// Size of the device scratch of ppf_apply_operator_noalloc (per GPU, per application
+  /// library).
+  constexpr std::size_t noalloc_scratch_bytes = std::size_t(PPF_NOALLOC_SCRATCH_BYTES);
+} // namespace ppf
+
+/// Device scratch of ppf_apply_operator_noalloc.  Each GPU the library runs on holds its
+/// own copy until the process ends.  In host code this name is only a stand-in for the
+/// symbol: kernels and copies take the device address from noalloc_scratch_ptr().
+static __device__ __align__(256) unsigned char scratch[ppf::noalloc_scratch_bytes];
+
+namespace ppf
+{
+  /// Device address of `scratch` on the current device, looked up once per device (with
+  /// that device current) and kept.
+  inline unsigned char *
+  noalloc_scratch_ptr()
+  {
+    constexpr int         max_devices = 64;
+    static unsigned char *ptr[max_devices] = {};
+    static std::mutex     lock;
+    int                   dev = 0;
+    PPF_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= max_devices)
+      throw Error(status_invalid, "device index out of range");
+    std::lock_guard<std::mutex> g(lock);
+    if (!ptr[dev])
+      {
+        void *p = nullptr;
+        PPF_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[dev] = static_cast<unsigned char *>(p);
+      }
+    return ptr[dev];
+  }
+
   template <class A, class = void>
   struct has_free_energy : std::false_type
   {};
@@ -35,8 +70,11 @@ namespace ppf
   {
     explicit Context(const std::string &prm)
       : rt(prm)
+      , scratch(noalloc_scratch_ptr())
     {}
     Runtime<PPF_APP> rt;
+    /// Device address of the scratch array on the context's device (set up at creation).
+    unsigned char   *scratch = nullptr;
     std::string      error;
     bool             synchronize = true;
     /// The adjoint driver of the application (created on first use), whose checkpoint
@@ -125,6 +163,70 @@ namespace ppf
       }
     else
       throw Error(status_unsupported, "this application declares no free energy for the adjoint");
+  }
+} // namespace ppf
+
+namespace ppf
+{
+  /// Sets the runtime's borrowed device region for the duration of one call.
+  template <class R>
+  struct BorrowScope
+  {
+    BorrowScope(R &rt_, unsigned char *base, int64_t bytes)
+      : rt(rt_)
+    {
+      rt.borrowed = {base, bytes, 0};
+    }
+    ~BorrowScope()
+    {
+      rt.borrowed = {};
+    }
+    R &rt;
+  };
+} // namespace ppf
+
+namespace ppf
+{
+  /// ppf_apply_operator's work on a context (shared by the allocation-free variant).
+  inline void
+  apply_operator_abi(Context &c, int block_id, int which, int scaling, int precision,
+                     const void *d_src, void *d_dst)
+  {
+    auto     &rt  = c.rt;
+    const int b   = rt.block_index(block_id);
+    const int nbf = int(rt.blocks[b].field_indices.size());
+    if (which < PPF_OP_RHS || which > PPF_OP_LHS_PLAIN || scaling < 0 || scaling > 2)
+      throw Error(status_invalid, "bad operator selector or scaling");
+    if (!d_dst || (which != PPF_OP_RHS && !d_src))
+      throw Error(status_invalid, "null device array");
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      auto lf = typename Runtime<PPF_APP>::template LevelFields<T> {};
+      std::vector<DeviceBuffer<T>> shadow;
+      if constexpr (std::is_same_v<T, double>)
+        lf = rt.fine_fields();
+      else
+        lf = rt.fp32_fields(b, which != PPF_OP_RHS, shadow); // FP32 copies of what it reads
+      std::vector<const T *> src;
+      std::vector<T *>       dst;
+      for (int k = 0; k < nbf; ++k)
+        {
+          if (d_src)
+            src.push_back(static_cast<const T *>(d_src) + k * rt.fine.n_nodes);
+          dst.push_back(static_cast<T *>(d_dst) + k * rt.fine.n_nodes);
+        }
+      if (which == PPF_OP_RHS)
+        src.clear();
+      rt.template apply_operator<T>(b, which != PPF_OP_RHS, lf, src, dst, scaling,
+                                    dirichlet_zero, which == PPF_OP_LHS_PLAIN);
+      // FP32 copies in context-owned buffers must outlive the launch.
+      if (c.synchronize || !shadow.empty())
+        rt.sync();
+    };
+    if (precision == PPF_FP32)
+      run(float {});
+    else
+      run(double {});
   }
 } // namespace ppf
 
@@ -372,40 +474,20 @@ ppf_apply_operator(ppf_ctx *ctx, int block_id, int which, int scaling, int preci
                    const void *d_src, void *d_dst)
 {
   return ppf::guarded(ctx, [&](ppf::Context &c) {
-    auto     &rt  = c.rt;
-    const int b   = rt.block_index(block_id);
-    const int nbf = int(rt.blocks[b].field_indices.size());
-    if (which < PPF_OP_RHS || which > PPF_OP_LHS_PLAIN || scaling < 0 || scaling > 2)
-      throw ppf::Error(ppf::status_invalid, "bad operator selector or scaling");
-    if (!d_dst || (which != PPF_OP_RHS && !d_src))
-      throw ppf::Error(ppf::status_invalid, "null device array");
-    auto run = [&](auto tag) {
-      using T    = decltype(tag);
-      auto lf    = typename ppf::Runtime<PPF_APP>::template LevelFields<T> {};
-      std::vector<ppf::DeviceBuffer<T>> shadow;
-      if constexpr (std::is_same_v<T, double>)
-        lf = rt.fine_fields();
-      else
-        lf = rt.fp32_fields(b, which != PPF_OP_RHS, shadow); // FP32 copies of what it reads
-      std::vector<const T *> src;
-      std::vector<T *>       dst;
-      for (int k = 0; k < nbf; ++k)
-        {
-          if (d_src)
-            src.push_back(static_cast<const T *>(d_src) + k * rt.fine.n_nodes);
-          dst.push_back(static_cast<T *>(d_dst) + k * rt.fine.n_nodes);
-        }
-      if (which == PPF_OP_RHS)
-        src.clear();
-      rt.template apply_operator<T>(b, which != PPF_OP_RHS, lf, src, dst, scaling,
-                                    ppf::dirichlet_zero, which == PPF_OP_LHS_PLAIN);
-      if (c.synchronize || !shadow.empty())
-        rt.sync();
-    };
-    if (precision == PPF_FP32)
-      run(float {});
-    else
-      run(double {});
+    ppf::apply_operator_abi(c, block_id, which, scaling, precision, d_src, d_dst);
+  });
+}
+
+int
+ppf_apply_operator_noalloc(ppf_ctx *ctx, int block_id, int which, int scaling, int precision,
+                           const void *d_src, void *d_dst)
+{
+  return ppf::guarded(ctx, [&](ppf::Context &c) {
+    if (!c.scratch)
+      throw ppf::Error(ppf::status_internal, "device scratch not set up");
+    ppf::BorrowScope<ppf::Runtime<PPF_APP>> scope(c.rt, c.scratch,
+                                                  int64_t(ppf::noalloc_scratch_bytes));
+    ppf::apply_operator_abi(c, block_id, which, scaling, precision, d_src, d_dst);
   });
 }
 

@@ -675,9 +675,16 @@ namespace ppf
           const int age = in.slot - 1;
           if (age < 0 || age >= int(storage[in.field].size()))
             throw Error(status_internal, "dependency storage not available");
-          shadow.emplace_back(fine.n_nodes);
-          convert(shadow.back().get(), storage[in.field][age].get(), fine.n_nodes);
-          lf.ptr[in.field][in.slot] = shadow.back().get();
+          float *copy = nullptr;
+          if (borrowed.base)
+            copy = carve<float>(fine.n_nodes, "the FP32 copies of the operator's inputs");
+          else
+            {
+              shadow.emplace_back(fine.n_nodes);
+              copy = shadow.back().get();
+            }
+          convert(copy, storage[in.field][age].get(), fine.n_nodes);
+          lf.ptr[in.field][in.slot] = copy;
         }
       return lf;
     }
@@ -830,9 +837,11 @@ namespace ppf
           any_array = out.dst[f];
       if (!any_array)
         throw Error(status_internal, "operator without an output field");
-      // An absent operand reads zeros, so its (unused) sums stay finite.
+      // An operator without gradient terms still runs with one operand, which reads
+      // zeros so that its (unused) sums stay finite; operands beyond the instantiated
+      // count are never read and point at any lattice-size array.
       for (int k = a.n_ops; k < stencil_max_ops; ++k)
-        a.op[k] = zero_array<number>(g.n_nodes);
+        a.op[k] = k == 0 ? zero_array<number>(g.n_nodes) : any_array;
       for (int f = 0; f < PPF_MAX_FIELDS; ++f)
         for (int s = 0; s < PPF_N_SLOTS; ++s)
           {
@@ -1153,11 +1162,45 @@ namespace ppf
                          out, scratch);
     }
 
+    /// Device region that replaces every allocation of the operator path while it is
+    /// set (ppf_apply_operator_noalloc): scratch, the zero operand and FP32 copies are
+    /// carved from it in call order; a request that does not fit throws.
+    struct BorrowedScratch
+    {
+      unsigned char *base  = nullptr;
+      int64_t        bytes = 0;
+      int64_t        used  = 0;
+    };
+    BorrowedScratch borrowed;
+
+    template <typename T>
+    T *
+    carve(int64_t n, const char *what)
+    {
+      const int64_t off  = (borrowed.used + 255) / 256 * 256;
+      const int64_t need = n * int64_t(sizeof(T));
+      if (off + need > borrowed.bytes)
+        throw Error(status_invalid, std::string(what) + " needs " + std::to_string(off + need) +
+                                      " bytes of device scratch; the allocation-free entry "
+                                      "point has " + std::to_string(borrowed.bytes));
+      borrowed.used = off + need;
+      return reinterpret_cast<T *>(borrowed.base + off);
+    }
+
     /// Device array of at least n zeros of type number (kept for the context).
     template <typename number>
     const number *
     zero_array(int64_t n)
     {
+      if (borrowed.base)
+        {
+          // The borrowed region is shared by every call on the GPU: clear it each time.
+          number *z = carve<number>(n, "the zero operand");
+          hipLaunchKernelGGL(fill_kernel<number>, dim3(grid_for(n, block_size)),
+                             dim3(block_size), 0, stream, z, number(0), n);
+          PPF_HIP_CHECK(hipGetLastError());
+          return z;
+        }
       auto &buf = [&]() -> DeviceBuffer<number> & {
         if constexpr (std::is_same_v<number, float>)
           return zeros_f;
@@ -1178,6 +1221,8 @@ namespace ppf
     number *
     ensure_scratch(int64_t n)
     {
+      if (borrowed.base)
+        return carve<number>(n, "the generic cell path's per-cell results");
       if constexpr (std::is_same_v<number, float>)
         {
           if (scratch_f.size() < n)
