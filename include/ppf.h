@@ -206,20 +206,28 @@ void *ppf_stream(ppf_ctx *ctx);
 
 int ppf_synchronize(ppf_ctx *ctx);
 
-/* Adjoint of the explicit Allen-Cahn step
- *   phi' = F(phi) = phi - dt*L*f'(phi) - dt*L*kappa * M^{-1} K phi
- * (an explicit Allen-Cahn block, reference applications/allen_cahn/explicit/
- * custom_pde.h:72-87, advanced by ExplicitSolver, explicit_solver.h:63-77) with the
- * application's free energy.  No reference counterpart: the reference has no adjoint.
- * Available in applications that declare a free energy for it; otherwise
- * PPF_ERR_UNSUPPORTED.  Arrays are device arrays in the stored layout.
+/* Adjoint of the application's explicit step.  No reference counterpart: the reference
+ * has no adjoint.  Available in applications that declare a free energy for it;
+ * otherwise PPF_ERR_UNSUPPORTED.  Arrays are device arrays in the stored layout; K is the
+ * assembled stiffness matrix and M the lumped mass.
+ *
+ * Allen-Cahn applications (an explicit Allen-Cahn block, reference applications/
+ * allen_cahn/explicit/custom_pde.h:72-87, advanced by ExplicitSolver,
+ * explicit_solver.h:63-77): phi is the order parameter, L the mobility and
+ *   phi' = F(phi) = phi - dt*L*f'(phi) - dt*L*kappa * M^{-1} K phi.
+ * Cahn-Hilliard applications (the c and mu blocks, reference applications/cahn_hilliard/
+ * explicit/custom_pde.h:92-118): phi is the concentration c, the parameter L is the
+ * mobility M_c, and with mu = f'(c) + kappa * M^{-1} K c (computed from the initial c
+ * and after every c update, as the mu block does)
+ *   c' = F(c) = c - dt*M_c * M^{-1} K mu.
  *
  * ppf_adjoint_forward: phi_final = F^n_steps(phi0) with parameters (L, kappa);
  * precision PPF_FP32 evaluates the step in FP32 (state kept in FP64).
  *
  * ppf_adjoint_vjp: given lam_final = dJ/dphi_N for any loss J, returns dJ/dL,
  * dJ/dkappa and dJ/dphi0, running the adjoint steps backwards in time and recomputing
- * forward states from checkpoints stored every `checkpoint_every` steps.  The
+ * forward states from checkpoints stored every `checkpoint_every` steps (Cahn-Hilliard:
+ * the segment keeps c and mu of every step).  The
  * checkpoints (ceil(n_steps / checkpoint_every) states) and the recomputed segment
  * (checkpoint_every states) stay allocated in the context for later calls, until
  * ppf_destroy.

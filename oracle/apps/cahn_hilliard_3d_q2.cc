@@ -11,6 +11,8 @@ struct OracleApp
   static constexpr int degree = 2;
   template <typename number>
   using PDE = prismspf::CustomPDE<dim, degree, number>;
+  using FreeEnergy                            = prismspf::CahnHilliardWell;
+  static constexpr bool cahn_hilliard_adjoint = true;
 
   static void
   define(std::vector<prismspf::FieldAttributes> &fields, std::vector<prismspf::SolveBlock> &blocks)

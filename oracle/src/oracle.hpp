@@ -60,6 +60,14 @@ namespace oracle
   template <class A>
   struct has_energy<A, std::void_t<typename A::FreeEnergy>> : std::true_type
   {};
+  /// Applications whose adjoint is the explicit Cahn-Hilliard step.
+  template <class A, class = void>
+  struct is_cahn_hilliard_adjoint : std::false_type
+  {};
+  template <class A>
+  struct is_cahn_hilliard_adjoint<A, std::void_t<decltype(A::cahn_hilliard_adjoint)>>
+    : std::bool_constant<A::cahn_hilliard_adjoint>
+  {};
 
   using prismspf::DependencyMap;
   using prismspf::SolveBlock;
@@ -2055,6 +2063,99 @@ namespace oracle
           lam = prev;
         }
       dphi0 = lam;
+    }
+
+    /// Explicit Cahn-Hilliard step (the c and mu blocks, reference applications/
+    /// cahn_hilliard/explicit/custom_pde.h:92-118, each an invm-scaled assembled
+    /// operator): mu = f'(c) + kappa M^-1 K c, c' = c - dt M_mob M^-1 K mu.
+    template <class E>
+    std::vector<double>
+    ch_potential(const std::vector<double> &c, double kappa) const
+    {
+      const auto          kc = stiffness(c);
+      std::vector<double> mu(top.ndofs, 0.0);
+      for (int64_t i = 0; i < top.ndofs; ++i)
+        if (cons[0].kind[i] == free_dof)
+          mu[i] = E::fprime(c[i]) + kappa * kc[i] / jxw[i];
+      distribute(cons[0], mu.data(), true);
+      return mu;
+    }
+
+    template <class E>
+    std::vector<double>
+    ch_step(const std::vector<double> &c, double mob, double kappa) const
+    {
+      const double        dt  = timer.get_timestep();
+      const auto          mu  = ch_potential<E>(c, kappa);
+      const auto          kmu = stiffness(mu);
+      std::vector<double> out(top.ndofs, 0.0);
+      for (int64_t i = 0; i < top.ndofs; ++i)
+        if (cons[0].kind[i] == free_dof)
+          out[i] = c[i] - dt * mob * kmu[i] / jxw[i];
+      distribute(cons[0], out.data(), true);
+      return out;
+    }
+
+    template <class E>
+    std::vector<double>
+    ch_forward(const std::vector<double> &c0, double mob, double kappa, unsigned n) const
+    {
+      auto c = c0;
+      for (unsigned s = 0; s < n; ++s)
+        c = ch_step<E>(c, mob, kappa);
+      return c;
+    }
+
+    /// Reverse sweep of the Cahn-Hilliard step storing every forward state:
+    /// lam_n = lam - dt M_mob (f''(c) w + kappa K(M^-1 w)) with w = K(M^-1 lam);
+    /// dJ/dM_mob += -dt lam . M^-1 K mu, dJ/dkappa += -dt M_mob w . M^-1 K c.
+    template <class E>
+    void
+    ch_vjp(const std::vector<double> &c0, std::vector<double> lam, double mob, double kappa,
+           unsigned n, double &dmob, double &dk, std::vector<double> &dc0) const
+    {
+      const double                     dt = timer.get_timestep();
+      std::vector<std::vector<double>> states {c0};
+      for (unsigned s = 0; s < n; ++s)
+        states.push_back(ch_step<E>(states.back(), mob, kappa));
+      for (int64_t i = 0; i < top.ndofs; ++i)
+        if (cons[0].kind[i] == image_dof)
+          {
+            lam[cons[0].target[i]] += lam[i];
+            lam[i] = 0.0;
+          }
+      auto over_mass = [&](const std::vector<double> &v) {
+        std::vector<double> o(top.ndofs, 0.0);
+        for (int64_t i = 0; i < top.ndofs; ++i)
+          if (cons[0].kind[i] == free_dof)
+            o[i] = v[i] / jxw[i];
+        distribute(cons[0], o.data(), true);
+        return o;
+      };
+      dmob = dk = 0.0;
+      for (int s = int(n) - 1; s >= 0; --s)
+        {
+          const auto &c   = states[s];
+          const auto  mu  = ch_potential<E>(c, kappa);
+          const auto  kmu = stiffness(mu);
+          const auto  kc  = stiffness(c);
+          auto        w   = stiffness(over_mass(lam));
+          for (int64_t i = 0; i < top.ndofs; ++i)
+            if (cons[0].kind[i] != free_dof)
+              w[i] = 0.0;
+          const auto          kw = stiffness(over_mass(w));
+          std::vector<double> prev(top.ndofs, 0.0);
+          for (int64_t i = 0; i < top.ndofs; ++i)
+            {
+              if (cons[0].kind[i] != free_dof)
+                continue;
+              prev[i] = lam[i] - dt * mob * (E::fprime2(c[i]) * w[i] + kappa * kw[i]);
+              dmob += lam[i] * (-dt * kmu[i] / jxw[i]);
+              dk += w[i] * (-dt * mob * kc[i] / jxw[i]);
+            }
+          lam = prev;
+        }
+      dc0 = lam;
     }
 
     // ----------------------------------------------------------------- state

@@ -272,8 +272,11 @@ oracle_adjoint_forward(void *h, const double *phi0, double L, double kappa, unsi
     using A = typename std::decay_t<decltype(m)>::AppType;
     if constexpr (oracle::has_energy<A>::value)
       {
-        std::vector<double> p(phi0, phi0 + m.top.ndofs);
-        auto r = m.template ac_forward<typename A::FreeEnergy>(p, L, kappa, n);
+        std::vector<double> p(phi0, phi0 + m.top.ndofs), r;
+        if constexpr (oracle::is_cahn_hilliard_adjoint<A>::value)
+          r = m.template ch_forward<typename A::FreeEnergy>(p, L, kappa, n);
+        else
+          r = m.template ac_forward<typename A::FreeEnergy>(p, L, kappa, n);
         std::copy(r.begin(), r.end(), out);
       }
     else
@@ -290,7 +293,10 @@ oracle_adjoint_vjp(void *h, const double *phi0, const double *lam, double L, dou
     if constexpr (oracle::has_energy<A>::value)
       {
         std::vector<double> p(phi0, phi0 + m.top.ndofs), l(lam, lam + m.top.ndofs), g;
-        m.template ac_vjp<typename A::FreeEnergy>(p, l, L, kappa, n, *dL, *dk, g);
+        if constexpr (oracle::is_cahn_hilliard_adjoint<A>::value)
+          m.template ch_vjp<typename A::FreeEnergy>(p, l, L, kappa, n, *dL, *dk, g);
+        else
+          m.template ac_vjp<typename A::FreeEnergy>(p, l, L, kappa, n, *dL, *dk, g);
         std::copy(g.begin(), g.end(), dphi0);
       }
     else

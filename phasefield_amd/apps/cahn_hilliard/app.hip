@@ -17,6 +17,9 @@ struct CahnHilliard3D
   template <typename number>
   using PDE      = prismspf::CustomPDE<dim, degree, number>;
   using BlockIds = std::integer_sequence<int, 0, 1, 2>;
+  /// Adjoint entry points: the explicit Cahn-Hilliard step with this free energy.
+  using FreeEnergy                            = prismspf::CahnHilliardWell;
+  static constexpr bool cahn_hilliard_adjoint = true;
 
   static const char *
   name()

@@ -14,6 +14,24 @@
 
 PRISMS_PF_BEGIN_NAMESPACE
 
+/// f(c) = c^2 (1-c)^2 and its derivatives f'(c) = 2c(1-c)(1-2c), f''(c) = 2 - 12c + 12c^2.
+struct CahnHilliardWell
+{
+  template <typename T>
+  PPF_HD static T
+  fprime(const T &c)
+  {
+    return 2.0 * c * (1.0 - c) * (1.0 - 2.0 * c);
+  }
+
+  template <typename T>
+  PPF_HD static T
+  fprime2(const T &c)
+  {
+    return 2.0 - 12.0 * c + 12.0 * c * c;
+  }
+};
+
 template <unsigned int dim, unsigned int degree, typename number>
 class CustomPDE : public PDEOperatorBase<dim, degree, number>
 {
@@ -73,7 +91,7 @@ private:
       {
         ScalarValue c  = variable_list.template get_value<Scalar, Current>(0);
         ScalarGrad  cx = variable_list.template get_gradient<Scalar, Current>(0);
-        variable_list.set_value_term(1, 2.0 * c * (1.0 - c) * (1.0 - 2.0 * c));
+        variable_list.set_value_term(1, CahnHilliardWell::fprime(c));
         variable_list.set_gradient_term(1, kappa * cx);
       }
     else if (solve_block_id == 2)

@@ -122,11 +122,25 @@ namespace ppf
       }
   }
 
+  /// Applications whose explicit step is the Cahn-Hilliard pair of blocks declare
+  /// `static constexpr bool cahn_hilliard_adjoint = true`; the others get the Allen-Cahn
+  /// adjoint.
+  template <class A, class = void>
+  struct is_cahn_hilliard_adjoint : std::false_type
+  {};
+  template <class A>
+  struct is_cahn_hilliard_adjoint<A, std::void_t<decltype(A::cahn_hilliard_adjoint)>>
+    : std::bool_constant<A::cahn_hilliard_adjoint>
+  {};
+
   template <class A>
   auto &
   adjoint_of(Context &c)
   {
-    using Adjoint = AllenCahnAdjoint<A::dim, A::degree, typename A::FreeEnergy>;
+    using Adjoint =
+      std::conditional_t<is_cahn_hilliard_adjoint<A>::value,
+                         CahnHilliardAdjoint<A::dim, A::degree, typename A::FreeEnergy>,
+                         AllenCahnAdjoint<A::dim, A::degree, typename A::FreeEnergy>>;
     if (!c.adjoint)
       c.adjoint = std::make_shared<Adjoint>(c.rt.fine, c.rt.stream);
     return *static_cast<Adjoint *>(c.adjoint.get());

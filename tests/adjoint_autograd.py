@@ -1,8 +1,10 @@
-"""torch.autograd.Function over the C-ABI adjoint of the explicit Allen-Cahn step.
+"""torch.autograd.Function over the C-ABI adjoint of an application's explicit step.
 
-`allen_cahn_steps(phi0, L, kappa, ctx, n_steps, checkpoint_every, precision)` returns
+`explicit_steps(phi0, L, kappa, ctx, n_steps, checkpoint_every, precision)` returns
 phi_N = F^n(phi0) computed by ppf_adjoint_forward; its backward calls ppf_adjoint_vjp
-with lam = dJ/dphi_N and returns dJ/dphi0, dJ/dL, dJ/dkappa.  Tensors are FP64 device
+with lam = dJ/dphi_N and returns dJ/dphi0, dJ/dL, dJ/dkappa.  F is the Allen-Cahn step of
+an Allen-Cahn application (L the mobility) or the Cahn-Hilliard step of a Cahn-Hilliard
+application (phi the concentration, L the mobility M).  Tensors are FP64 device
 tensors in the stored layout (periodic images omitted).  Test harness only.
 """
 
@@ -11,7 +13,7 @@ from __future__ import annotations
 import torch
 
 
-class AllenCahnSteps(torch.autograd.Function):
+class ExplicitSteps(torch.autograd.Function):
     @staticmethod
     def forward(fctx, phi0, L, kappa, ctx, n_steps, checkpoint_every, precision):
         phi0 = phi0.detach().contiguous()
@@ -36,8 +38,8 @@ class AllenCahnSteps(torch.autograd.Function):
                 torch.tensor(dk, dtype=kappa.dtype, device=kappa.device), None, None, None, None)
 
 
-def allen_cahn_steps(phi0, L, kappa, ctx, n_steps, checkpoint_every, precision):
-    return AllenCahnSteps.apply(phi0, L, kappa, ctx, n_steps, checkpoint_every, precision)
+def explicit_steps(phi0, L, kappa, ctx, n_steps, checkpoint_every, precision):
+    return ExplicitSteps.apply(phi0, L, kappa, ctx, n_steps, checkpoint_every, precision)
 
 
 def lumped_mass(ctx):

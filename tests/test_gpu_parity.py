@@ -351,6 +351,56 @@ def test_adjoint_matches_oracle(device, app, text, precision=FP64):
         assert rel_err(d_g.cpu().numpy(), to_stored(rg, info)) <= 1e-10
 
 
+def _ch_adj_text(periodic, subdivisions=None, dt=1e-3):
+    # dt below the explicit stability limit ~ h^4 / (M kappa) of the finest axis
+    text = prm.cahn_hilliard(size=8.0, refinement=2, steps=0, dt=dt, subdivisions=subdivisions)
+    return text if periodic else text.replace("periodic = true", "periodic = false")
+
+
+CH_ADJ_CASES = [_ch_adj_text(True), _ch_adj_text(False, [2, 1, 3], dt=2e-5)]
+
+
+@pytest.mark.parametrize("text", CH_ADJ_CASES, ids=["3d_periodic", "3d_natural_2x1x3"])
+def test_cahn_hilliard_adjoint_matches_oracle(device, text):
+    """Adjoint of the explicit Cahn-Hilliard step (c, mu blocks) against the oracle's:
+    forward within 1e-12, dJ/dM, dJ/dkappa and dJ/dc0 within 1e-10, for several
+    checkpoint spacings."""
+    app = "cahn_hilliard_3d_q2"
+    ctx, orc = Context(app, text), Oracle(app, text)
+    info = ctx.info
+    rng = np.random.default_rng(6)
+    c0 = 0.5 + rng.uniform(-0.2, 0.2, info.n_nodes)
+    lam = rng.uniform(-1, 1, info.n_nodes)
+    n, mob, kappa = 8, 1.1, 1.3
+    d_c0, d_out = dev(c0), torch.zeros(info.n_nodes, dtype=torch.float64, device="cuda")
+    ctx.adjoint_forward(d_c0.data_ptr(), mob, kappa, n, FP64, d_out.data_ptr())
+    ref = orc.adjoint_forward(to_full(c0, info), mob, kappa, n)
+    assert rel_err(d_out.cpu().numpy(), to_stored(ref, info)) <= 1e-12
+    rdm, rdk, rg = orc.adjoint_vjp(to_full(c0, info), _zero_images(lam, info), mob, kappa, n)
+    for every in (1, 3, 8, 50):
+        d_lam, d_g = dev(lam), torch.zeros(info.n_nodes, dtype=torch.float64, device="cuda")
+        dm, dk = ctx.adjoint_vjp(d_c0.data_ptr(), d_lam.data_ptr(), mob, kappa, n, every, FP64,
+                                 d_g.data_ptr())
+        print("ch adjoint", every, dm, rdm, dk, rdk, flush=True)
+        assert abs(dm - rdm) <= 1e-10 * abs(rdm)
+        assert abs(dk - rdk) <= 1e-10 * abs(rdk)
+        assert rel_err(d_g.cpu().numpy(), to_stored(rg, info)) <= 1e-10
+
+
+def test_cahn_hilliard_explicit_matches_adjoint_forward(device):
+    """The Cahn-Hilliard adjoint's forward map equals the application's c and mu blocks."""
+    n = 4
+    text = prm.cahn_hilliard(size=8.0, refinement=3, steps=n, dt=1e-3)
+    ctx = Context("cahn_hilliard_3d_q2", text)
+    ctx.run(0)
+    c0 = to_stored(ctx.get_field(0, AGE_LATEST).ravel(), ctx.info)
+    ctx.run(n)
+    cn = to_stored(ctx.get_field(0, AGE_LATEST).ravel(), ctx.info)
+    d0, dn = dev(c0), torch.zeros(ctx.info.n_nodes, dtype=torch.float64, device="cuda")
+    ctx.adjoint_forward(d0.data_ptr(), 1.0, 1.0, n, FP64, dn.data_ptr())
+    assert rel_err(dn.cpu().numpy(), cn) <= 1e-12
+
+
 def _zero_images(stored, info):
     full = to_full(stored, info).reshape(tuple(reversed([info.degree * info.cells[d] + 1 for d in range(info.dim)])))
     for ax in range(info.dim):

@@ -377,6 +377,58 @@ def test_adjoint_gradients_match_finite_differences(oracle_libs):
     assert abs(dphi @ v - fd_v) <= 1e-6 * abs(fd_v)
 
 
+@pytest.mark.parametrize("periodic", [True, False], ids=["periodic", "natural"])
+def test_cahn_hilliard_adjoint_matches_finite_differences(oracle_libs, periodic):
+    """The oracle's Cahn-Hilliard adjoint (c' = c - dt M L mu, mu = f'(c) + kappa L c):
+    dJ/dM, dJ/dkappa and a directional derivative of dJ/dc0 against central finite
+    differences of its forward map, J = 1/2 sum m (c_N - target)^2."""
+    size, ref, n, dt = 8.0, 2, 10, 1e-3
+    text = prm.cahn_hilliard(size=size, refinement=ref, dt=dt, steps=n, outputs=0)
+    if not periodic:
+        text = text.replace("periodic = true", "periodic = false")
+    o = oracle("cahn_hilliard_3d_q2", text)
+    mesh = bf.Mesh(3, 2, [2 ** ref] * 3, [size] * 3, [periodic] * 3)
+    m = o.lumped_mass()
+    X = mesh.node_x()
+    c0 = bf.distribute(mesh, 0.5 + 0.3 * np.sin(2 * np.pi * X[:, 0] / size)
+                       * np.cos(2 * np.pi * X[:, 1] / size) * np.cos(2 * np.pi * X[:, 2] / size))
+    target = o.adjoint_forward(c0, 1.2, 1.3, n)
+
+    def J(mob, kappa, p0):
+        r = o.adjoint_forward(p0, mob, kappa, n) - target
+        return 0.5 * float(m @ (r * r))
+
+    mob, kappa = 1.0, 1.5
+    lam = m * (o.adjoint_forward(c0, mob, kappa, n) - target)
+    dmob, dk, dc = o.adjoint_vjp(c0, lam, mob, kappa, n)
+    eps = 1e-6
+    fd_m = (J(mob + eps, kappa, c0) - J(mob - eps, kappa, c0)) / (2 * eps)
+    fd_k = (J(mob, kappa + eps, c0) - J(mob, kappa - eps, c0)) / (2 * eps)
+    rng = np.random.default_rng(8)
+    v = bf.distribute(mesh, rng.standard_normal(mesh.ndofs))
+    fd_v = (J(mob, kappa, c0 + eps * v) - J(mob, kappa, c0 - eps * v)) / (2 * eps)
+    assert abs(dmob - fd_m) <= 1e-6 * abs(fd_m), (dmob, fd_m)
+    assert abs(dk - fd_k) <= 1e-6 * abs(fd_k), (dk, fd_k)
+    assert abs(dc @ v - fd_v) <= 1e-6 * abs(fd_v), (dc @ v, fd_v)
+
+
+def test_cahn_hilliard_adjoint_forward_is_the_explicit_blocks(oracle_libs):
+    """The adjoint's forward map is the application's c and mu blocks: n increments of
+    the explicit run from the same c0 give the same c (1e-13), and the lumped mass
+    sum m c is conserved (periodic)."""
+    n = 6
+    text = prm.cahn_hilliard(size=8.0, refinement=2, dt=1e-3, steps=n, outputs=0)
+    o = oracle("cahn_hilliard_3d_q2", text)
+    o.run(0)  # increment 0: the application's initial condition and mu from it
+    c0 = o.get_field(0, -1)
+    o.run(n)
+    cn = o.get_field(0, -1)
+    fwd = o.adjoint_forward(c0, 1.0, 1.0, n)  # the application's M = kappa = 1
+    assert rel(fwd, cn) <= 1e-13
+    m = o.lumped_mass()
+    assert abs(m @ fwd - m @ c0) <= 1e-13 * abs(m @ c0)
+
+
 @pytest.mark.parametrize("periodic", [False, True], ids=["natural", "periodic"])
 def test_newton_matches_dense_newton(oracle_libs, periodic):
     """One implicit Allen-Cahn increment: the oracle's Newton loop against a dense Newton
