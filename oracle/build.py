@@ -31,6 +31,7 @@ APPS = [
     "poisson_2d_q2",
     "poisson_3d_q2",
     "precipitate_3d_q1",
+    "switching_2d_q2",
     "testbed_2d_q2",
     "testbed_3d_q1",
 ]
@@ -59,10 +60,34 @@ def compile_app(app: str) -> Path:
     return out
 
 
+# CPU baseline of the Cahn-Hilliard increment (baseline/ch_baseline.cc): one build per
+# vector ISA; oracle/baseline.py loads the one the host supports.
+BASELINE_VARIANTS = {"avx512": "-march=x86-64-v4", "avx2": "-march=x86-64-v3"}
+
+
+def baseline_path(variant: str) -> Path:
+    return BUILD / f"libch_baseline_{variant}.so"
+
+
+def compile_baseline(variant: str) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    out = baseline_path(variant)
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [CXX, "-O3", BASELINE_VARIANTS[variant], "-std=c++20", "-fopenmp", "-fPIC", "-shared",
+           str(HERE / "baseline" / "ch_baseline.cc"), "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"baseline build ({variant}) failed:\n{res.stderr[-4000:]}")
+    os.replace(tmp, out)
+    return out
+
+
 def build(apps: list[str] | None = None) -> list[Path]:
     apps = apps or APPS
     with ThreadPoolExecutor(max_workers=min(8, len(apps))) as pool:
-        return list(pool.map(compile_app, apps))
+        out = list(pool.map(compile_app, apps))
+    out += [compile_baseline(v) for v in BASELINE_VARIANTS]
+    return out
 
 
 if __name__ == "__main__":

@@ -57,6 +57,30 @@ namespace dealii
   template <typename Number>
   VectorizedArray<Number> make_vectorized_array(Number u) { return u; }
 
+  // deal.II's lane-wise select (vectorization.h).
+  enum class SIMDComparison : int { equal, not_equal, less_than, less_than_or_equal, greater_than, greater_than_or_equal };
+
+  template <SIMDComparison p, typename N>
+  N compare_and_apply_mask(const N &l, const N &r, const N &t, const N &f)
+  {
+    switch (p)
+      {
+        case SIMDComparison::equal: return l == r ? t : f;
+        case SIMDComparison::not_equal: return l != r ? t : f;
+        case SIMDComparison::less_than: return l < r ? t : f;
+        case SIMDComparison::less_than_or_equal: return l <= r ? t : f;
+        case SIMDComparison::greater_than: return l > r ? t : f;
+        default: return l >= r ? t : f;
+      }
+  }
+
+  template <SIMDComparison p, typename N, std::size_t w>
+  VectorizedArray<N, w> compare_and_apply_mask(const VectorizedArray<N, w> &l, const VectorizedArray<N, w> &r,
+                                               const VectorizedArray<N, w> &t, const VectorizedArray<N, w> &f)
+  {
+    return compare_and_apply_mask<p, N>(l.v, r.v, t.v, f.v);
+  }
+
 } // namespace dealii
 
 namespace std

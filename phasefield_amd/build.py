@@ -31,6 +31,7 @@ APPS = {
     "poisson_2d_q2": "apps/poisson/app_2d.hip",
     "poisson_3d_q2": "apps/poisson/app_3d.hip",
     "precipitate_3d_q1": "apps/precipitate/app.hip",
+    "switching_2d_q2": "apps/switching/app_2d.hip",
     "testbed_2d_q2": "apps/testbed/app_2d_q2.hip",
     "testbed_3d_q1": "apps/testbed/app_3d_q1.hip",
 }

@@ -110,9 +110,10 @@ namespace ppf
       number volume = number(1);
     };
 
-    /// Evaluates the hook; returns the container with the submitted terms.
+    /// Evaluates the hook at simulated time `time` and increment `inc`; returns the
+    /// container with the submitted terms.
     FC
-    eval(const State &st, double time, double dt) const
+    eval(const State &st, double time, double dt, unsigned int inc) const
     {
       FC fc;
       for (int f = 0; f < PPF_MAX_FIELDS; ++f)
@@ -128,6 +129,7 @@ namespace ppf
       fc.clear_terms();
       fc.clear_reads();
       prismspf::SimulationTimer timer(dt, time);
+      timer.set_increment(inc);
       const prismspf::PDEOperatorBase<dim, degree, number> &base = pde;
       if (lhs)
         base.compute_lhs(fc, timer, block.id);
@@ -137,14 +139,15 @@ namespace ppf
       return fc;
     }
 
+    /// A random input state whose values lie in [centre - width, centre + width].
     State
-    random_state(std::mt19937_64 &rng) const
+    random_state(std::mt19937_64 &rng, double centre, double width) const
     {
       std::uniform_real_distribution<double> u(-1.0, 1.0);
       State                                  st;
       for (const auto &in : inputs)
         {
-          st.v[in.field][in.slot] = number(0.5 + 0.4 * u(rng));
+          st.v[in.field][in.slot] = number(centre + width * u(rng));
           for (int d = 0; d < dim; ++d)
             st.g[in.field][in.slot][d] = number(u(rng));
         }
@@ -154,8 +157,15 @@ namespace ppf
       return st;
     }
 
+    /// Structure of the hook at simulated time `time` and increment `inc`.  The random
+    /// states cover values in [0.1, 0.9] (order parameters, concentrations), [-3, 3] and
+    /// [-50, 50], so a coefficient that branches on a value outside the unit interval
+    /// is seen as non-constant; the coefficients are compared at several times and
+    /// increments (a plan whose coefficients change with them is re-derived whenever
+    /// either changes).  A branch the probe still misses is caught by the runtime's
+    /// comparison with the generic path at the plan's first use on real fields.
     SplitPlan
-    analyse(double time, double dt) const
+    analyse(double time, double dt, unsigned int inc) const
     {
       SplitPlan        plan;
       std::mt19937_64  rng(0x9e3779b97f4a7c15ull ^ (uint64_t(block.id) * 2 + lhs));
@@ -167,11 +177,15 @@ namespace ppf
       };
 
       std::vector<State> states;
-      for (int k = 0; k < 6; ++k)
-        states.push_back(random_state(rng));
+      for (int k = 0; k < 3; ++k)
+        states.push_back(random_state(rng, 0.5, 0.4));
+      for (int k = 0; k < 3; ++k)
+        states.push_back(random_state(rng, 0.0, 3.0));
+      for (int k = 0; k < 2; ++k)
+        states.push_back(random_state(rng, 0.0, 50.0));
       std::vector<FC> base;
       for (const auto &st : states)
-        base.push_back(eval(st, time, dt));
+        base.push_back(eval(st, time, dt, inc));
       for (const auto &fc : base)
         if (fc.geometry_reads)
           return fail("the hook reads the quadrature-point location or element volume");
@@ -189,8 +203,8 @@ namespace ppf
                 st2.g[in.field][in.slot][d] = number(2.0 * u(rng));
                 st0.g[in.field][in.slot][d] = number(0);
               }
-          const FC a = eval(st2, time, dt);
-          const FC z = eval(st0, time, dt);
+          const FC a = eval(st2, time, dt, inc);
+          const FC z = eval(st0, time, dt, inc);
           for (int f = 0; f < PPF_MAX_FIELDS; ++f)
             {
               if (a.value_term[f] != base[k].value_term[f])
@@ -212,7 +226,7 @@ namespace ppf
             for (int d = 0; d < dim; ++d)
               st.g[o.field][o.slot][d] = number(0);
           st.g[in.field][in.slot][0] = number(1);
-          const FC r                 = eval(st, time, dt);
+          const FC r                 = eval(st, time, dt, inc);
           for (int f = 0; f < PPF_MAX_FIELDS; ++f)
             {
               if (!(out_mask & (1u << f)))
@@ -254,13 +268,24 @@ namespace ppf
               }
         }
 
-      // Time dependence of the coefficients.
+      // Dependence of the coefficients on the simulated time or the increment.
       {
-        const FC a = eval(states[1], time + 0.731, dt);
-        for (int f = 0; f < PPF_MAX_FIELDS; ++f)
-          for (int d = 0; d < dim; ++d)
-            if (a.gradient_term[f][d] != base[1].gradient_term[f][d])
-              plan.time_dependent = true;
+        const double       times[] = {time + 0.731, time + 13.7, 2.0 * time + 101.0};
+        const unsigned int incs[]  = {inc + 1, inc + 3, 3u, 1000u};
+        for (const State *st : {&states[1], &states[4]})
+          {
+            const FC &ref = base[st == &states[1] ? 1 : 4];
+            auto      cmp = [&](const FC &a) {
+              for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+                for (int d = 0; d < dim; ++d)
+                  if (a.gradient_term[f][d] != ref.gradient_term[f][d])
+                    plan.time_dependent = true;
+            };
+            for (const double t : times)
+              cmp(eval(*st, t, dt, inc));
+            for (const unsigned int i : incs)
+              cmp(eval(*st, time, dt, i));
+          }
       }
 
       // Constant derivative of the value term with respect to the own src value.
@@ -277,9 +302,9 @@ namespace ppf
               s0.v[f][src] = number(0);
               s1.v[f][src] = number(1);
               s2.v[f][src] = number(2);
-              const double r0 = double(eval(s0, time, dt).value_term[f]);
-              const double r1 = double(eval(s1, time, dt).value_term[f]);
-              const double r2 = double(eval(s2, time, dt).value_term[f]);
+              const double r0 = double(eval(s0, time, dt, inc).value_term[f]);
+              const double r1 = double(eval(s1, time, dt, inc).value_term[f]);
+              const double r2 = double(eval(s2, time, dt, inc).value_term[f]);
               const double a  = r1 - r0;
               if (std::abs((r2 - r1) - a) > tol * (std::abs(a) + std::abs(r2)) + 1e-300 ||
                   (k > 0 && std::abs(a - first) > tol * std::abs(first) + 1e-300))

@@ -603,28 +603,44 @@ namespace ppf
 
     struct PlanEntry
     {
-      SplitPlan plan;
-      double    time = 0.0;
-      double    dt   = 0.0;
+      SplitPlan    plan;
+      double       time      = 0.0;
+      double       dt        = 0.0;
+      unsigned int increment = 0;
+      /// The split path was compared with the generic path on real fields (first use).
+      bool verified = false;
     };
 
+    /// The split plan of block b's rhs or lhs, re-derived when the time step changes and,
+    /// for coefficients that depend on the time or the increment, when either changes.
     template <typename number>
-    const SplitPlan &
+    PlanEntry &
     split_plan(int b, bool lhs)
     {
       const OpKey key {b, lhs, std::is_same_v<number, float> ? 1 : 0};
       auto        it = plans.find(key);
-      if (it != plans.end() && !(it->second.plan.time_dependent &&
-                                 it->second.time != timer.get_time()) &&
-          it->second.dt == timer.get_timestep())
-        return it->second.plan;
+      if (it != plans.end() && it->second.dt == timer.get_timestep() &&
+          !(it->second.plan.time_dependent &&
+            (it->second.time != timer.get_time() ||
+             it->second.increment != timer.get_increment())))
+        return it->second;
       const auto &pde = *pde_of<number>();
       HookProbe<PDE<number>, dim, degree, number> probe(pde, blocks[b], lhs);
       PlanEntry entry;
-      entry.plan = probe.analyse(timer.get_time(), timer.get_timestep());
-      entry.time = timer.get_time();
-      entry.dt   = timer.get_timestep();
-      return plans[key] = entry, plans[key].plan;
+      entry.plan      = probe.analyse(timer.get_time(), timer.get_timestep(), timer.get_increment());
+      entry.time      = timer.get_time();
+      entry.dt        = timer.get_timestep();
+      entry.increment = timer.get_increment();
+      // A plan that was rejected by the runtime comparison stays rejected.
+      if (it != plans.end() && !it->second.plan.eligible && it->second.verified)
+        {
+          entry.plan.eligible = false;
+          entry.plan.reason   = it->second.plan.reason;
+          entry.verified      = true;
+        }
+      else if (it != plans.end())
+        entry.verified = it->second.verified && entry.plan.eligible;
+      return plans[key] = entry, plans[key];
     }
 
     template <typename number>
@@ -751,48 +767,141 @@ namespace ppf
       out.dirichlet_write = dwrite;
       out.halt            = enqueue_halt;
 
-      const SplitPlan &plan = split_plan<number>(b, lhs);
-      const bool use_split  = path == OperatorPath::Split || path == OperatorPath::SplitNode ||
-                             (path == OperatorPath::Auto && plan.eligible);
+      PlanEntry       &entry = split_plan<number>(b, lhs);
+      const SplitPlan &plan  = entry.plan;
+      bool use_split = path == OperatorPath::Split || path == OperatorPath::SplitNode ||
+                       (path == OperatorPath::Auto && plan.eligible);
       if (use_split && !plan.eligible)
         throw Error(status_unsupported, "split path requested but the hook does not "
                                         "qualify: " + plan.reason);
       const auto        &pde = *pde_of<number>();
       const unsigned int nb  = grid_for(g.n_nodes, block_size);
-      dispatch_block_id(typename App::BlockIds {}, blk.id, [&](auto bid) {
-        constexpr int id = decltype(bid)::value;
-        if (use_split && path != OperatorPath::SplitNode &&
-            launch_stencil<number, id>(b, lhs, g, deps, out, plan))
-          return;
-        if (use_split)
-          {
-            if (lhs)
-              hipLaunchKernelGGL((split_operator_kernel<PDE<number>, dim, degree, number, true, id>),
-                                 dim3(nb), dim3(block_size), 0, stream, pde, g, deps, out,
-                                 plan.terms, timer);
-            else
-              hipLaunchKernelGGL((split_operator_kernel<PDE<number>, dim, degree, number, false, id>),
-                                 dim3(nb), dim3(block_size), 0, stream, pde, g, deps, out,
-                                 plan.terms, timer);
-          }
-        else
-          {
-            number *scratch = ensure_scratch<number>(int64_t(bf.size()) *
-                                                     dofs_per_cell<dim, degree> * g.n_cells);
-            const unsigned int nc = grid_for(g.n_cells, block_size);
-            if (lhs)
-              hipLaunchKernelGGL(
-                (cell_operator_kernel<PDE<number>, dim, degree, number, true, id, false, App::n_fields>),
-                dim3(nc), dim3(block_size), 0, stream, pde, g, deps, out, scratch, timer, -1);
-            else
-              hipLaunchKernelGGL(
-                (cell_operator_kernel<PDE<number>, dim, degree, number, false, id, false, App::n_fields>),
-                dim3(nc), dim3(block_size), 0, stream, pde, g, deps, out, scratch, timer, -1);
-            hipLaunchKernelGGL((node_gather_kernel<dim, degree, number>), dim3(nb),
-                               dim3(block_size), 0, stream, g, out, scratch);
-          }
-      });
-      PPF_HIP_CHECK(hipGetLastError());
+      auto launch = [&](bool split, const OutTable<number> &o) {
+        dispatch_block_id(typename App::BlockIds {}, blk.id, [&](auto bid) {
+          constexpr int id = decltype(bid)::value;
+          if (split && path != OperatorPath::SplitNode &&
+              launch_stencil<number, id>(b, lhs, g, deps, o, plan))
+            return;
+          if (split)
+            {
+              if (lhs)
+                hipLaunchKernelGGL((split_operator_kernel<PDE<number>, dim, degree, number, true, id>),
+                                   dim3(nb), dim3(block_size), 0, stream, pde, g, deps, o,
+                                   plan.terms, timer);
+              else
+                hipLaunchKernelGGL((split_operator_kernel<PDE<number>, dim, degree, number, false, id>),
+                                   dim3(nb), dim3(block_size), 0, stream, pde, g, deps, o,
+                                   plan.terms, timer);
+            }
+          else
+            {
+              number *scratch = ensure_scratch<number>(int64_t(bf.size()) *
+                                                       dofs_per_cell<dim, degree> * g.n_cells);
+              const unsigned int nc = grid_for(g.n_cells, block_size);
+              if (lhs)
+                hipLaunchKernelGGL(
+                  (cell_operator_kernel<PDE<number>, dim, degree, number, true, id, false, App::n_fields>),
+                  dim3(nc), dim3(block_size), 0, stream, pde, g, deps, o, scratch, timer, -1);
+              else
+                hipLaunchKernelGGL(
+                  (cell_operator_kernel<PDE<number>, dim, degree, number, false, id, false, App::n_fields>),
+                  dim3(nc), dim3(block_size), 0, stream, pde, g, deps, o, scratch, timer, -1);
+              hipLaunchKernelGGL((node_gather_kernel<dim, degree, number>), dim3(nb),
+                                 dim3(block_size), 0, stream, g, o, scratch);
+            }
+        });
+        PPF_HIP_CHECK(hipGetLastError());
+      };
+
+      // First use of an automatically chosen split plan on the fine lattice: the split
+      // path's result is compared with the generic path's on the real fields, and the
+      // plan is dropped (for good, with a line in the output log) if they disagree.  This
+      // catches coefficients that branch on values or times the structure probe did not
+      // visit.
+      if (use_split && path == OperatorPath::Auto && !entry.verified && lf.grid == &fine &&
+          !std::any_of(std::begin(dst), std::end(dst),
+                       [&](const number *d) {
+                         for (const number *sp : src)
+                           if (sp == d)
+                             return true;
+                         return false;
+                       }))
+        {
+          const int64_t n   = g.n_nodes;
+          number       *ref = nullptr;
+          if (borrowed.base)
+            {
+              if (borrowed.used + int64_t(bf.size()) * n * int64_t(sizeof(number)) + 256 +
+                    int64_t(bf.size()) * dofs_per_cell<dim, degree> * g.n_cells *
+                      int64_t(sizeof(number)) + 256 <=
+                  borrowed.bytes)
+                ref = carve<number>(int64_t(bf.size()) * n, "the split-path comparison");
+            }
+          else
+            {
+              auto &buf = [&]() -> DeviceBuffer<number> & {
+                if constexpr (std::is_same_v<number, float>)
+                  return verify_f;
+                else
+                  return verify_d;
+              }();
+              if (buf.size() < int64_t(bf.size()) * n)
+                buf.resize(int64_t(bf.size()) * n);
+              ref = buf.get();
+            }
+          if (ref)
+            {
+              OutTable<number> oref = out;
+              for (std::size_t k = 0; k < bf.size(); ++k)
+                oref.dst[bf[k]] = ref + int64_t(k) * n;
+              launch(false, oref);
+              launch(true, out);
+              double worst = 0.0, scale = 0.0;
+              for (std::size_t k = 0; k < bf.size(); ++k)
+                {
+                  hipLaunchKernelGGL(max_diff_partial_kernel<number>, dim3(reduce_blocks),
+                                     dim3(reduce_threads), 0, stream, reduce_buf.get(),
+                                     dst[k], oref.dst[bf[k]], n);
+                  std::vector<double> part(2 * reduce_blocks);
+                  PPF_HIP_CHECK(hipMemcpyAsync(part.data(), reduce_buf.get(),
+                                               sizeof(double) * part.size(),
+                                               hipMemcpyDeviceToHost, stream));
+                  sync();
+                  for (int i = 0; i < reduce_blocks; ++i)
+                    {
+                      worst = std::max(worst, part[i]);
+                      scale = std::max(scale, part[reduce_blocks + i]);
+                    }
+                }
+              int halted_now = 0;
+              if (out.halt)
+                PPF_HIP_CHECK(hipMemcpy(&halted_now, out.halt, sizeof(int),
+                                        hipMemcpyDeviceToHost));
+              if (!halted_now)
+                {
+                  const double tol = std::is_same_v<number, float> ? 1e-4 : 1e-11;
+                  entry.verified   = true;
+                  if (!(worst <= tol * scale) && !(scale == 0.0 && worst == 0.0))
+                    {
+                      std::ostringstream msg;
+                      msg << "solve block " << blk.id << " (" << (lhs ? "lhs" : "rhs")
+                          << "): the split operator differs from the generic path by "
+                          << worst / std::max(scale, 1e-300)
+                          << " (relative) on the current fields; using the generic path";
+                      entry.plan.eligible = false;
+                      entry.plan.reason   = msg.str();
+                      log += msg.str() + "\n";
+                      if (std::getenv("PPF_VERBOSE"))
+                        std::fprintf(stderr, "%s\n", msg.str().c_str());
+                      // the generic result is the operator: copy it into place
+                      for (std::size_t k = 0; k < bf.size(); ++k)
+                        copy(dst[k], oref.dst[bf[k]], n);
+                    }
+                }
+              return;
+            }
+        }
+      launch(use_split, out);
     }
 
     /// Launches the tiled stencil kernel for a split operator; false if the operator
@@ -1092,7 +1201,7 @@ namespace ppf
       const prismspf::SolveBlock &blk = blocks[b];
       const GridDesc             &g   = *lf.grid;
       std::vector<int>            bf(blk.field_indices.begin(), blk.field_indices.end());
-      const SplitPlan            &plan = split_plan<number>(b, true);
+      const SplitPlan            &plan = split_plan<number>(b, true).plan;
       const bool                  use_split =
         path != OperatorPath::Generic && plan.eligible && plan.alpha_constant;
       const unsigned int nb = grid_for(g.n_nodes, block_size);
@@ -2959,6 +3068,9 @@ namespace ppf
     std::map<OpKey, PlanEntry>                     plans;
     DeviceBuffer<double>                           scratch_d;
     DeviceBuffer<float>                            scratch_f;
+    /// Reference output of the split-path comparison (first use of a plan).
+    DeviceBuffer<double>                           verify_d;
+    DeviceBuffer<float>                            verify_f;
     DeviceBuffer<double>                           zeros_d;
     DeviceBuffer<float>                            zeros_f;
     DeviceBuffer<double>                           reduce_buf;

@@ -42,6 +42,50 @@ namespace ppf
     return r;
   }
 
+  /// Block-level maximum of one double per thread; valid in thread 0.
+  __device__ PPF_INLINE double
+  block_max(double v)
+  {
+    __shared__ double partial[reduce_threads / 32];
+    for (int o = warpSize / 2; o > 0; o /= 2)
+      v = fmax(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x % warpSize;
+    const int wave = threadIdx.x / warpSize;
+    if (lane == 0)
+      partial[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+      for (int w = 0; w < int(blockDim.x) / warpSize; ++w)
+        r = fmax(r, partial[w]);
+    return r;
+  }
+
+  /// Per-block max |a - b| (NaN or infinite counted as +inf) and max |b|, into
+  /// partial[b] and partial[gridDim.x + b].
+  template <typename T>
+  __global__ void __launch_bounds__(reduce_threads)
+  max_diff_partial_kernel(double *__restrict__ partial, const T *__restrict__ a,
+                          const T *__restrict__ b, int64_t n)
+  {
+    double d = 0.0, m = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+      {
+        const double e = fabs(double(a[i]) - double(b[i]));
+        d              = e <= d ? d : (e == e && e < INFINITY ? e : INFINITY);
+        m              = fmax(m, fabs(double(b[i])));
+      }
+    d = block_max(d);
+    __syncthreads();
+    m = block_max(m);
+    if (threadIdx.x == 0)
+      {
+        partial[blockIdx.x]             = d;
+        partial[gridDim.x + blockIdx.x] = m;
+      }
+  }
+
   template <typename T>
   __global__ void
   fill_kernel(T *__restrict__ x, T value, int64_t n)

@@ -537,6 +537,51 @@ namespace std
 
 namespace dealii
 {
+  /// Lane-wise comparisons (deal.II vectorization.h, SIMDComparison and
+  /// compare_and_apply_mask): for each lane, true_value where `left predicate right`
+  /// holds, false_value otherwise.
+  enum class SIMDComparison : int
+  {
+    equal,
+    not_equal,
+    less_than,
+    less_than_or_equal,
+    greater_than,
+    greater_than_or_equal
+  };
+
+  template <SIMDComparison predicate, typename Number>
+  PPF_HD PPF_INLINE Number
+  compare_and_apply_mask(const Number &left, const Number &right, const Number &true_value,
+                         const Number &false_value)
+  {
+    bool c = false;
+    if constexpr (predicate == SIMDComparison::equal)
+      c = left == right;
+    else if constexpr (predicate == SIMDComparison::not_equal)
+      c = left != right;
+    else if constexpr (predicate == SIMDComparison::less_than)
+      c = left < right;
+    else if constexpr (predicate == SIMDComparison::less_than_or_equal)
+      c = left <= right;
+    else if constexpr (predicate == SIMDComparison::greater_than)
+      c = left > right;
+    else
+      c = left >= right;
+    return c ? true_value : false_value;
+  }
+
+  template <SIMDComparison predicate, typename Number, std::size_t width>
+  PPF_HD PPF_INLINE VectorizedArray<Number, width>
+  compare_and_apply_mask(const VectorizedArray<Number, width> &left,
+                         const VectorizedArray<Number, width> &right,
+                         const VectorizedArray<Number, width> &true_value,
+                         const VectorizedArray<Number, width> &false_value)
+  {
+    return compare_and_apply_mask<predicate, Number>(left[0], right[0], true_value[0],
+                                                     false_value[0]);
+  }
+
   template <int dim, typename Number>
   PPF_HD PPF_INLINE Number
   Tensor<1, dim, Number>::norm() const

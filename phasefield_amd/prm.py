@@ -261,6 +261,20 @@ def mechanics(size=100.0, refinement=6, E=2.0, nu=0.3, tolerance=1e-10,
     return "\n".join(lines) + "\n"
 
 
+def switching(size=8.0, refinement=4, steps=6, dt=2e-3, D=1.0, c_low=-10.0, c_high=70.0,
+              outputs=1):
+    """Routing test bed (apps/switching): the initial c ramps from c_low to c_high along
+    x, so the value branches of its flux coefficients (c > 1, c > 60) are taken in part of
+    the domain."""
+    return "\n".join([
+        mesh_block(2, size, refinement, periodic=False),
+        _common(steps, dt, outputs),
+        f"set Model constant D = {D!r}, DOUBLE",
+        f"set Model constant c_low = {c_low!r}, DOUBLE",
+        f"set Model constant c_high = {c_high!r}, DOUBLE",
+    ]) + "\n"
+
+
 def config_prm(name: str, **kw) -> str:
     """Parameter text of a benchmark configuration (keyword overrides shrink it)."""
     if name == "allen_cahn_2d":
