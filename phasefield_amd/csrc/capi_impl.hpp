@@ -357,6 +357,12 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
       c.rt.stencil_waves = std::stoi(v);
     else if (k == "stencil_z_chunk")
       c.rt.stencil_z_chunk = std::stoi(v);
+    else if (k == "stencil_map")
+      {
+        if (v != "linear" && v != "band")
+          throw ppf::Error(ppf::status_invalid, "stencil_map: linear or band");
+        c.rt.stencil_map = v == "band" ? 1 : 0;
+      }
     else if (k == "stencil_target_blocks")
       c.rt.stencil_target_blocks = std::max(1, std::stoi(v));
     else if (k == "stencil_store_policy")

@@ -1113,7 +1113,11 @@ namespace ppf
           a.z_chunk  = 1;
           a.chunks_z = 1;
         }
-      const int64_t nblocks = int64_t(a.tiles_x) * a.tiles_y * a.chunks_z;
+      int64_t nblocks = int64_t(a.tiles_x) * a.tiles_y * a.chunks_z;
+      a.map_mode      = stencil_map;
+      a.ty_per_xcd    = (a.tiles_y + 7) / 8;
+      if (a.map_mode == 1)
+        nblocks = 8 * int64_t(a.tiles_x) * a.ty_per_xcd * a.chunks_z;
       const int     wg_threads = stencil_tx * waves;
       const int64_t nrem       = (a.rem_a + a.rem_b + wg_threads - 1) / wg_threads;
       if (nblocks + nrem >= (int64_t(1) << 31))
@@ -3041,6 +3045,8 @@ namespace ppf
     int                                    stencil_rows    = 0;
     int                                    stencil_waves   = 0;
     int                                    stencil_z_chunk = 0;
+    /// Workgroup -> tile order of the stencil kernel (StencilArgs::map_mode).
+    int                                    stencil_map     = 0;
     int                                    stencil_target_blocks = 2048;
     /// Cache policy of the stencil kernel's output stores (Auto: non-temporal for
     /// launches whose arrays exceed stencil_nt_min_bytes).

@@ -57,6 +57,12 @@ namespace ppf
     int    n[3];        // nodes per axis (n[2] = 1 for 2-D lattices)
     int    periodic[3];
     int    tiles_x, tiles_y, chunks_z, z_chunk, n_blocks;
+    /// Workgroup -> tile order (workgroups are dealt round-robin to the 8 XCDs).
+    /// 0: consecutive tiles (x fastest, then y, then z chunk) on one XCD;
+    /// 1: XCD k owns the tile rows [k ty_per_xcd, (k+1) ty_per_xcd) of every z chunk and
+    ///    walks them chunk by chunk, so a tile's y and z neighbours run on its XCD
+    ///    (n_blocks = 8 x the largest per-XCD count; surplus workgroups exit).
+    int    map_mode, ty_per_xcd;
     /// Tiles cover x < tile_n[0], y < tile_n[1]; the rest of the lattice (a few columns
     /// or rows left over by the tile widths) is processed node by node by the
     /// workgroups after the n_blocks tile workgroups: box A = [tile_n[0], n0) x [0, n1)
@@ -371,13 +377,29 @@ namespace ppf
         return;
       }
 
-    const int b   = blockIdx.x;
-    const int q8  = a.n_blocks / 8;
-    const int lb  = b < 8 * q8 ? (b % 8) * q8 + b / 8 : b;
-    const int t_x = lb % a.tiles_x;
-    const int t_r = lb / a.tiles_x;
-    const int t_y = t_r % a.tiles_y;
-    const int chk = t_r / a.tiles_y;
+    const int b = blockIdx.x;
+    int       t_x, t_y, chk;
+    if (a.map_mode == 1)
+      {
+        const int xcd = b % 8, i = b / 8;
+        const int ty0 = xcd * a.ty_per_xcd;
+        const int cnt = min(a.ty_per_xcd, a.tiles_y - ty0);
+        if (cnt <= 0 || i >= a.tiles_x * cnt * a.chunks_z)
+          return;
+        t_x         = i % a.tiles_x;
+        const int r = i / a.tiles_x;
+        t_y         = ty0 + r % cnt;
+        chk         = r / cnt;
+      }
+    else
+      {
+        const int q8  = a.n_blocks / 8;
+        const int lb  = b < 8 * q8 ? (b % 8) * q8 + b / 8 : b;
+        t_x           = lb % a.tiles_x;
+        const int t_r = lb / a.tiles_x;
+        t_y           = t_r % a.tiles_y;
+        chk           = t_r / a.tiles_y;
+      }
     const int x0  = t_x * stencil_tx;
     const int y0  = t_y * TY;
     const int zb  = dim == 3 ? chk * a.z_chunk : 0; // even for Q2 (host)
