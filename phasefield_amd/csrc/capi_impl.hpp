@@ -363,6 +363,14 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
           throw ppf::Error(ppf::status_invalid, "stencil_map: linear or band");
         c.rt.stencil_map = v == "band" ? 1 : 0;
       }
+    else if (k == "generic_kernel")
+      {
+        if (v != "cell_block" && v != "two_pass")
+          throw ppf::Error(ppf::status_invalid, "generic_kernel: cell_block or two_pass");
+        c.rt.generic_cell_block = v == "cell_block";
+      }
+    else if (k == "generic_column_chunk")
+      c.rt.column_chunk = std::max(0, std::stoi(v));
     else if (k == "stencil_target_blocks")
       c.rt.stencil_target_blocks = std::max(1, std::stoi(v));
     else if (k == "stencil_store_policy")

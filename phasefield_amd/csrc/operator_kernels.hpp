@@ -389,6 +389,646 @@ namespace ppf
   }
 
   // ------------------------------------------------------------------------------
+  // Generic path, one launch: cell blocks in shared memory
+  // ------------------------------------------------------------------------------
+  //
+  // A workgroup evaluates a box of cells and writes the final values of the nodes it
+  // owns.  Per evaluated box:
+  //   1. every input (field, slot) of the operator is staged in shared memory over the
+  //      box's nodes (per-axis tables give each position's global offset, periodic wrap
+  //      and Dirichlet faces; constrained inputs read as zero on their Dirichlet faces);
+  //   2. one thread per cell walks the (p+1)^d collocated quadrature points with the
+  //      hook inlined, reading values and gradients from the staged box (1-D derivative
+  //      contractions), and integrates the submitted terms into the cell's (p+1)^d
+  //      nodal contributions, kept in registers;
+  //   3. per output field the contributions go to shared memory, and every owned node
+  //      sums those of its <= 2^d cells in the fixed order of node_gather_kernel (lower
+  //      cell before upper cell, z outermost), applies the scaling and the Dirichlet
+  //      rule, and is written once.
+  // Two kernels use these steps:
+  //   * cell_block_kernel (1-D, 2-D): a block of E cells per axis owning the nodes of
+  //     its last E - 1; the first layer is the neighbour whose upper faces carry owned
+  //     nodes, evaluated again;
+  //   * cell_column_kernel (3-D): a 9 x 9 column of cells owning 8 x 8 marches up a
+  //     chunk of z layers in slabs of 3.  The nodes on a slab's top plane keep their
+  //     lower-cell sums in shared memory until the next slab adds the upper cells, so
+  //     only the chunk's first layer and the x/y halo are evaluated twice.
+  // No global scratch, one launch, deterministic: the sum of every node is formed in
+  // the same order as by the two-pass kernels, so both give the same bits.
+
+  constexpr int cb_threads    = 256;
+  constexpr int cb_max_inputs = PPF_MAX_FIELDS * PPF_N_SLOTS;
+
+  /// Evaluated cells per axis of a cell_block_kernel block (their product is cb_threads).
+  template <int dim>
+  constexpr int cb_extent(int d)
+  {
+    return dim == 3 ? (d == 2 ? 4 : 8) : (dim == 2 ? (d < 2 ? 16 : 1) : (d == 0 ? 256 : 1));
+  }
+
+  /// Evaluated cells per slab of cell_column_kernel: 9 x 9 x 3 (243 of the 256 threads).
+  constexpr int cc_ex = 9, cc_ey = 9, cc_ez = 3;
+
+  /// One owned lattice position of a block along one axis.
+  struct GatherAxis
+  {
+    int          a[2];  ///< shared-memory offsets of the slot 0 / slot 1 cell contributions
+    unsigned int slots; ///< bit s: slot s exists
+    unsigned int faces; ///< Dirichlet face bits of the position
+    int64_t      goff;  ///< global node index part (position * stride); -1: not owned
+    double       mass;  ///< 1-D lumped mass (1 beyond dim)
+  };
+
+  struct CellBlockArgs
+  {
+    int n_in;
+    int in_field[cb_max_inputs];
+    int in_slot[cb_max_inputs];
+    int nblk[3];  ///< blocks (cell_block_kernel) or x/y tiles and z chunks (column kernel)
+    int bcell[3]; ///< owned cells per block and axis (column kernel: 8, 8, chunk layers)
+    /// Staged position of each (field, slot), 0 for those not staged (never read).
+    int pos[PPF_MAX_FIELDS][PPF_N_SLOTS];
+  };
+
+  namespace cellblock
+  {
+    constexpr unsigned long long off_mask = (1ull << 48) - 1;
+    constexpr unsigned long long outside  = 1ull << 63;
+
+    /// Staging table entry of lattice position `gi` along axis d (d < dim): global
+    /// offset (wrapped when periodic), Dirichlet face bits << 48, or the outside bit.
+    PPF_HD PPF_INLINE unsigned long long
+    staging_entry(const GridDesc &g, int d, int gi)
+    {
+      const int nd = g.nodes[d];
+      unsigned long long e;
+      if (g.periodic[d])
+        {
+          gi = ((gi % nd) + nd) % nd;
+          e  = 0;
+        }
+      else if (gi < 0 || gi >= nd)
+        return outside;
+      else
+        e = (gi == 0 ? 1ull << (48 + 2 * d) : 0ull) |
+            (gi == nd - 1 ? 1ull << (48 + 2 * d + 1) : 0ull);
+      const int64_t stride =
+        d == 0 ? 1 : (d == 1 ? int64_t(g.nodes[0]) : int64_t(g.nodes[0]) * g.nodes[1]);
+      return e | (unsigned long long)(int64_t(gi) * stride);
+    }
+
+    /// Gather table entry of owned lattice position n along axis d (d < dim) of a box
+    /// whose first evaluated cell is c0: slot offsets in the contribution array (cell
+    /// stride cstr, local-node stride kstr), face bits, global offset, 1-D mass.
+    template <int degree>
+    PPF_HD PPF_INLINE GatherAxis
+    gather_entry(const GridDesc &g, int d, int n, bool owned, int c0, int cstr, int kstr)
+    {
+      GatherAxis e {};
+      const int64_t stride =
+        d == 0 ? 1 : (d == 1 ? int64_t(g.nodes[0]) : int64_t(g.nodes[0]) * g.nodes[1]);
+      e.goff       = owned ? int64_t(n) * stride : -1;
+      const int lp = n - degree * c0; // lattice position in the box, >= degree
+      if (degree == 2 && (n & 1))
+        {
+          e.a[0]  = ((lp - 1) / 2) * cstr + kstr;
+          e.a[1]  = 0;
+          e.slots = 1;
+        }
+      else
+        {
+          const int  right     = n / degree;
+          const bool has_left  = right > 0 || g.periodic[d];
+          const bool has_right = right < g.cells[d];
+          e.a[0]  = has_left ? (lp / degree - 1) * cstr + degree * kstr : (lp / degree) * cstr;
+          e.a[1]  = (lp / degree) * cstr;
+          e.slots = (has_left && has_right) ? 3u : 1u;
+        }
+      e.faces = g.periodic[d] ? 0u
+                              : ((n == 0 ? 1u << (2 * d) : 0u) |
+                                 (n == g.nodes[d] - 1 ? 1u << (2 * d + 1) : 0u));
+      e.mass = owned ? lumped_mass_1d<degree>(g, d, n) : 1.0;
+      return e;
+    }
+
+    /// Stages every input over a box of bx x by x bz nodes whose per-axis staging
+    /// entries are in `table`; per batch of up to four inputs every load is issued
+    /// before the first shared-memory store, so their latencies overlap.
+    template <int bx, int by, int bz, int ax_max, typename number>
+    __device__ PPF_INLINE void
+    stage(number *lds, const unsigned long long (*table)[ax_max], const DepTable<number> &deps,
+          const CellBlockArgs &cb, int tid)
+    {
+      constexpr int box_n      = bx * by * bz;
+      constexpr int per_thread = (box_n + cb_threads - 1) / cb_threads;
+      constexpr int batch      = 4;
+      for (int s0 = 0; s0 < cb.n_in; s0 += batch)
+        {
+          number v[batch][per_thread];
+#pragma unroll
+          for (int u = 0; u < batch; ++u)
+            {
+              if (s0 + u >= cb.n_in)
+                break;
+              const int                f   = cb.in_field[s0 + u];
+              const int                sl  = cb.in_slot[s0 + u];
+              const number            *src = deps.ptr[f][sl];
+              const unsigned long long dm =
+                (deps.flags[f][sl] & slot_constrained) ? deps.dirichlet_mask[f] : 0u;
+              const unsigned long long reject = (dm << 48) | outside;
+#pragma unroll
+              for (int r = 0; r < per_thread; ++r)
+                {
+                  const int i  = tid + r * cb_threads;
+                  const int ii = i < box_n ? i : 0;
+                  const unsigned long long e0 = table[0][ii % bx];
+                  const unsigned long long e1 = table[1][(ii / bx) % by];
+                  const unsigned long long e2 = table[2][ii / (bx * by)];
+                  const bool keep = i < box_n && !((e0 | e1 | e2) & reject);
+                  const unsigned long long o =
+                    (e0 & off_mask) + (e1 & off_mask) + (e2 & off_mask);
+                  // rejected nodes read entry 0 (in bounds) and keep zero
+                  const number x = src[keep ? o : 0ull];
+                  v[u][r]        = keep ? x : number(0);
+                }
+            }
+#pragma unroll
+          for (int u = 0; u < batch; ++u)
+            {
+              if (s0 + u >= cb.n_in)
+                break;
+#pragma unroll
+              for (int r = 0; r < per_thread; ++r)
+                {
+                  const int i = tid + r * cb_threads;
+                  if (i < box_n)
+                    lds[(s0 + u) * box_n + i] = v[u][r];
+                }
+            }
+        }
+    }
+
+    /// Step 2 for one cell: the hook at every collocated point of cell `gc` (global
+    /// cell coordinates), whose local node (0,0,0) is at `base` of every staged box
+    /// (strides bx, bx by), integrated into acc.
+    template <class PDE, int dim, int degree, typename number, bool lhs, int block_id>
+    __device__ PPF_INLINE void
+    integrate_cell(const PDE &pde, const GridDesc &g, const prismspf::SimulationTimer &timer,
+                   const number *lds, int box_n, const CellBlockArgs &cb, int base, int sx,
+                   int sxy, const int (&gc)[3],
+                   number (&acc)[PPF_MAX_FIELDS][dofs_per_cell<dim, degree>])
+    {
+      constexpr int NPC = dofs_per_cell<dim, degree>;
+      constexpr int P1  = degree + 1;
+      using Basis       = GLL<degree>;
+      prismspf::FieldContainer<dim, degree, number> fc;
+      fc.block_view = true;
+      // Constant indices into the container only: a runtime index would place the
+      // whole container in private memory.
+#pragma unroll
+      for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+#pragma unroll
+        for (int sl = 0; sl < PPF_N_SLOTS; ++sl)
+          fc.block_ptr[f][sl] = lds + cb.pos[f][sl] * box_n;
+      fc.block_base      = base;
+      fc.block_stride[0] = sx;
+      fc.block_stride[1] = sxy;
+
+      number inv_h[dim];
+      number cell_volume = number(1);
+#pragma unroll
+      for (int d = 0; d < dim; ++d)
+        {
+          inv_h[d] = number(1.0 / g.h[d]);
+          cell_volume *= number(g.h[d]);
+        }
+      fc.element_volume = cell_volume;
+#pragma unroll
+      for (int d = 0; d < dim; ++d)
+        fc.inv_h[d] = inv_h[d];
+
+#pragma unroll
+      for (int q = 0; q < NPC; ++q)
+        {
+          number weight = cell_volume;
+#pragma unroll
+          for (int d = 0; d < dim; ++d)
+            weight *= number(Basis::w[prismspf::local_coord<degree>(q, d)]);
+          fc.cell_q = q;
+#pragma unroll
+          for (int d = 0; d < dim; ++d)
+            fc.q_point[d] =
+              number(g.lower[d] + (degree * gc[d] + prismspf::local_coord<degree>(q, d)) *
+                                    (g.h[d] / degree));
+          fc.clear_terms();
+          call_hook<PDE, dim, degree, number, lhs, block_id>(pde, fc, timer);
+#pragma unroll
+          for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+            {
+              if (fc.value_term_mask & (1u << f))
+                acc[f][q] += weight * fc.value_term[f];
+              if (fc.gradient_term_mask & (1u << f))
+                {
+#pragma unroll
+                  for (int d = 0; d < dim; ++d)
+                    {
+                      const number t  = weight * inv_h[d] * fc.gradient_term[f][d];
+                      const int    qd = prismspf::local_coord<degree>(q, d);
+#pragma unroll
+                      for (int k = 0; k < P1; ++k)
+                        if (Basis::D[qd][k] != 0.0)
+                          acc[f][prismspf::with_coord<degree>(q, d, k)] +=
+                            number(Basis::D[qd][k]) * t;
+                    }
+                }
+            }
+        }
+    }
+
+    /// Global cell coordinate of evaluated cell c0 + e along axis d: wrapped when
+    /// periodic; `valid` cleared when outside a non-periodic axis.
+    PPF_HD PPF_INLINE int
+    cell_coordinate(const GridDesc &g, int d, int c, bool &valid)
+    {
+      if (g.periodic[d])
+        return ((c % g.cells[d]) + g.cells[d]) % g.cells[d];
+      valid = valid && c >= 0 && c < g.cells[d];
+      return c;
+    }
+
+    /// Writes an owned node: the scaled sum, zero at a Dirichlet node, nothing at a
+    /// Dirichlet node of an explicit update (dirichlet_faces_kernel writes those
+    /// values afterwards; the kernels read no global memory while storing, so no store
+    /// waits for an earlier one).
+    template <typename number>
+    PPF_HD PPF_INLINE void
+    finish_node(const OutTable<number> &out, int f, number sum, unsigned int faces,
+                double mass, int64_t node)
+    {
+      if (faces & out.dirichlet_mask[f])
+        {
+          if (out.dirichlet_write != dirichlet_value)
+            out.dst[f][node] = number(0);
+          return;
+        }
+      double scale = 1.0;
+      if (out.scaling != scale_none)
+        scale = out.scaling == scale_invm ? 1.0 / mass : 1.0 / sqrt(mass);
+      out.dst[f][node] = number(scale * double(sum));
+    }
+  } // namespace cellblock
+
+  template <class PDE, int dim, int degree, typename number, bool lhs, int block_id>
+  __global__ void __launch_bounds__(cb_threads)
+  cell_block_kernel(const PDE pde, const GridDesc g, const DepTable<number> deps,
+                    const OutTable<number> out, const prismspf::SimulationTimer timer,
+                    const CellBlockArgs cb)
+  {
+    constexpr int NPC = dofs_per_cell<dim, degree>;
+    constexpr int P1  = degree + 1;
+    constexpr int E0 = cb_extent<dim>(0), E1 = cb_extent<dim>(1), E2 = cb_extent<dim>(2);
+    static_assert(E0 * E1 * E2 == cb_threads, "one evaluated cell per thread");
+    extern __shared__ unsigned char cb_shared[];
+    number *lds = reinterpret_cast<number *>(cb_shared);
+
+    if (halted(out.halt))
+      return;
+    const int tid = threadIdx.x;
+    int       bc[3];
+    {
+      int b = blockIdx.x;
+      bc[0] = b % cb.nblk[0];
+      b /= cb.nblk[0];
+      bc[1] = b % cb.nblk[1];
+      bc[2] = b / cb.nblk[1];
+    }
+    // First evaluated cell per axis: one layer below the owned cells.
+    int c0[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      c0[d] = d < dim ? bc[d] * cb.bcell[d] - 1 : 0;
+    constexpr int bx    = degree * E0 + 1;
+    constexpr int by    = dim > 1 ? degree * E1 + 1 : 1;
+    constexpr int bz    = dim > 2 ? degree * E2 + 1 : 1;
+    constexpr int box_n = bx * by * bz;
+    constexpr int ax_max = bx > by ? (bx > bz ? bx : bz) : (by > bz ? by : bz);
+    // Owned lattice positions per axis: p B (+1: the closing boundary node).
+    constexpr int ox     = degree * (E0 - 1) + 1;
+    constexpr int oy     = dim > 1 ? degree * (E1 - 1) + 1 : 1;
+    constexpr int oz     = dim > 2 ? degree * (E2 - 1) + 1 : 1;
+    constexpr int ow_max = ox > oy ? (ox > oz ? ox : oz) : (oy > oz ? oy : oz);
+    constexpr int n_cand = (ox * oy * oz + cb_threads - 1) / cb_threads;
+    __shared__ unsigned long long axis_table[3][ax_max];
+    __shared__ GatherAxis         gather_table[3][ow_max];
+    {
+      constexpr int ext[3] = {bx, by, bz};
+      int           d = 0, l = tid;
+      while (d < 3 && l >= ext[d])
+        l -= ext[d++];
+      if (d < 3)
+        axis_table[d][l] = d < dim ? cellblock::staging_entry(g, d, degree * c0[d] + l) : 0ull;
+    }
+    {
+      constexpr int ext[3]  = {ox, oy, oz};
+      constexpr int cstr[3] = {1, E0, E0 * E1};
+      constexpr int kstr[3] = {cb_threads, P1 * cb_threads, P1 * P1 * cb_threads};
+      int           d = 0, j = tid;
+      while (d < 3 && j >= ext[d])
+        j -= ext[d++];
+      if (d < 3)
+        {
+          GatherAxis e {};
+          if (d < dim)
+            {
+              const int n0 = degree * bc[d] * cb.bcell[d];
+              const int n1 = (bc[d] + 1) * cb.bcell[d] >= g.cells[d]
+                               ? g.nodes[d]
+                               : degree * (bc[d] + 1) * cb.bcell[d];
+              e = cellblock::gather_entry<degree>(g, d, n0 + j, n0 + j < n1, c0[d], cstr[d],
+                                                  kstr[d]);
+            }
+          else
+            {
+              e.slots = 1;
+              e.mass  = 1.0;
+            }
+          gather_table[d][j] = e;
+        }
+    }
+    __syncthreads();
+
+    // 1. Stage the inputs over the node box.
+    cellblock::stage<bx, by, bz, ax_max>(lds, axis_table, deps, cb, tid);
+    __syncthreads();
+
+    // 2. This thread's cell.
+    const int ec[3] = {tid % E0, (tid / E0) % E1, tid / (E0 * E1)};
+    int       gc[3];
+    bool      valid = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      gc[d] = d < dim ? cellblock::cell_coordinate(g, d, c0[d] + ec[d], valid) : 0;
+    number acc[PPF_MAX_FIELDS][NPC];
+#pragma unroll
+    for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+#pragma unroll
+      for (int k = 0; k < NPC; ++k)
+        acc[f][k] = number(0);
+    if (valid)
+      cellblock::integrate_cell<PDE, dim, degree, number, lhs, block_id>(
+        pde, g, timer, lds, box_n, cb,
+        degree * ec[0] + bx * (degree * ec[1] + by * degree * ec[2]), bx, bx * by, gc, acc);
+
+    // 3. Per output field: contributions to shared memory, owned nodes gathered.  Per
+    //    axis a table gives the two cell slots' shared-memory offsets (an absent slot
+    //    reads a zero entry, which leaves the sum unchanged), the 1-D lumped mass, the
+    //    Dirichlet face bits and the global index part.
+    __syncthreads(); // every cell evaluated: the staged inputs are no longer read
+    constexpr int zero_at = NPC * cb_threads;
+#pragma unroll
+    for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+      {
+        if (out.scratch_slot[f] < 0)
+          continue;
+#pragma unroll
+        for (int k = 0; k < NPC; ++k)
+          lds[k * cb_threads + tid] = acc[f][k];
+        if (tid == 0)
+          lds[zero_at] = number(0);
+        __syncthreads();
+        for (int c = 0; c < n_cand; ++c)
+          {
+            const int        i  = tid + c * cb_threads;
+            const int        ii = i < ox * oy * oz ? i : 0;
+            const GatherAxis t0 = gather_table[0][ii % ox];
+            const GatherAxis t1 = gather_table[1][(ii / ox) % oy];
+            const GatherAxis t2 = gather_table[2][ii / (ox * oy)];
+            if (!((i < ox * oy * oz) & (t0.goff >= 0) & (t1.goff >= 0) & (t2.goff >= 0)))
+              continue;
+            number sum = number(0);
+#pragma unroll
+            for (int a2 = 0; a2 < 2; ++a2)
+#pragma unroll
+              for (int a1 = 0; a1 < 2; ++a1)
+#pragma unroll
+                for (int a0 = 0; a0 < 2; ++a0)
+                  {
+                    const unsigned int present = (t0.slots >> a0) & (t1.slots >> a1) &
+                                                 (t2.slots >> a2) & 1u;
+                    sum += lds[present ? t0.a[a0] + t1.a[a1] + t2.a[a2] : zero_at];
+                  }
+            cellblock::finish_node(out, f, sum, t0.faces | t1.faces | t2.faces,
+                                   t0.mass * t1.mass * t2.mass, t0.goff + t1.goff + t2.goff);
+          }
+        __syncthreads(); // contributions of this field no longer read
+      }
+  }
+
+  /// 3-D generic kernel: a column of 9 x 9 cells (owning the nodes of its last 8 x 8)
+  /// marching up a chunk of z layers [z0, z0 + chunk) in slabs of 3 layers.  The first
+  /// slab starts one layer below the chunk.  Per slab, node planes p zs .. p zs + 3p:
+  /// the bottom plane adds the upper cells to the sums kept from the previous slab (in
+  /// shared memory, `carry`), the top plane's lower-cell sums are kept for the next
+  /// slab, and the planes between are complete.  Planes outside the chunk's
+  /// [p z0, p z1) (plus the closing boundary plane of a non-periodic last chunk) are
+  /// not written.  Grid: x tiles * y tiles * z chunks.
+  template <class PDE, int degree, typename number, bool lhs, int block_id>
+  __global__ void __launch_bounds__(cb_threads)
+  cell_column_kernel(const PDE pde, const GridDesc g, const DepTable<number> deps,
+                     const OutTable<number> out, const prismspf::SimulationTimer timer,
+                     const CellBlockArgs cb)
+  {
+    constexpr int dim = 3;
+    constexpr int NPC = dofs_per_cell<dim, degree>;
+    constexpr int P1  = degree + 1;
+    constexpr int bx = degree * cc_ex + 1, by = degree * cc_ey + 1, bz = degree * cc_ez + 1;
+    constexpr int box_n  = bx * by * bz;
+    constexpr int ax_max = bx > bz ? bx : bz;
+    constexpr int ox = degree * (cc_ex - 1) + 1, oy = degree * (cc_ey - 1) + 1;
+    constexpr int n_cols  = ox * oy;
+    constexpr int zero_at = NPC * cb_threads;
+    constexpr int kz      = P1 * P1 * cb_threads;  // local-node stride along z
+    constexpr int cz      = cc_ex * cc_ey;         // cell stride along z
+    extern __shared__ unsigned char cb_shared[];
+    number *lds = reinterpret_cast<number *>(cb_shared);
+    const int region = (cb.n_in * box_n > zero_at + 1 ? cb.n_in * box_n : zero_at + 1);
+    number   *carry  = lds + region; // [output slot][column]
+
+    if (halted(out.halt))
+      return;
+    const int tid   = threadIdx.x;
+    const int tile  = blockIdx.x % (cb.nblk[0] * cb.nblk[1]);
+    const int chunk = blockIdx.x / (cb.nblk[0] * cb.nblk[1]);
+    const int tx = tile % cb.nblk[0], ty = tile / cb.nblk[0];
+    const int c0x = tx * (cc_ex - 1) - 1, c0y = ty * (cc_ey - 1) - 1;
+    const int z0  = chunk * cb.bcell[2];
+    const int z1  = min(z0 + cb.bcell[2], g.cells[2]);
+    const int pz0 = degree * z0, pz1 = degree * z1;
+    // the closing boundary plane belongs to the last chunk of a non-periodic z axis
+    const int  pz_close = (!g.periodic[2] && z1 == g.cells[2]) ? g.nodes[2] - 1 : INT_MIN;
+    const int64_t zstride = int64_t(g.nodes[0]) * g.nodes[1];
+
+    __shared__ unsigned long long axis_table[3][ax_max];
+    __shared__ GatherAxis         gather_table[2][ox > oy ? ox : oy];
+    {
+      // x/y staging and gather entries, fixed for the whole column
+      if (tid < bx)
+        axis_table[0][tid] = cellblock::staging_entry(g, 0, degree * c0x + tid);
+      else if (tid < bx + by)
+        axis_table[1][tid - bx] = cellblock::staging_entry(g, 1, degree * c0y + tid - bx);
+      const int j = tid - (bx + by);
+      if (j >= 0 && j < ox + oy)
+        {
+          const int d  = j < ox ? 0 : 1;
+          const int jj = j < ox ? j : j - ox;
+          const int t  = d == 0 ? tx : ty;
+          const int n0 = degree * t * (cc_ex - 1);
+          const int n1 = (t + 1) * (cc_ex - 1) >= g.cells[d] ? g.nodes[d]
+                                                             : degree * (t + 1) * (cc_ex - 1);
+          gather_table[d][jj] = cellblock::gather_entry<degree>(
+            g, d, n0 + jj, n0 + jj < n1, d == 0 ? c0x : c0y, d == 0 ? 1 : cc_ex,
+            d == 0 ? cb_threads : P1 * cb_threads);
+        }
+    }
+
+    const int ec[3] = {tid % cc_ex, (tid / cc_ex) % cc_ey, tid / (cc_ex * cc_ey)};
+    for (int zs = z0 - 1; zs < z1; zs += cc_ez)
+      {
+        const bool first = zs == z0 - 1;
+        if (tid < bz)
+          axis_table[2][tid] = cellblock::staging_entry(g, 2, degree * zs + tid);
+        __syncthreads(); // tables written; the previous slab's contributions read
+
+        // 1. Stage the slab's node box.
+        cellblock::stage<bx, by, bz, ax_max>(lds, axis_table, deps, cb, tid);
+        __syncthreads();
+
+        // 2. This thread's cell.
+        number acc[PPF_MAX_FIELDS][NPC];
+#pragma unroll
+        for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+#pragma unroll
+          for (int k = 0; k < NPC; ++k)
+            acc[f][k] = number(0);
+        {
+          bool valid = tid < cc_ex * cc_ey * cc_ez;
+          int  gc[3];
+          gc[0] = cellblock::cell_coordinate(g, 0, c0x + ec[0], valid);
+          gc[1] = cellblock::cell_coordinate(g, 1, c0y + ec[1], valid);
+          gc[2] = cellblock::cell_coordinate(g, 2, zs + ec[2], valid);
+          if (valid)
+            cellblock::integrate_cell<PDE, dim, degree, number, lhs, block_id>(
+              pde, g, timer, lds, box_n, cb,
+              degree * ec[0] + bx * (degree * ec[1] + by * degree * ec[2]), bx, bx * by, gc,
+              acc);
+        }
+        __syncthreads(); // every cell evaluated: the staged inputs are no longer read
+
+        // 3. Per output field: contributions, then every owned column of the tile walks
+        //    the slab's node planes.
+#pragma unroll
+        for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+          {
+            const int slot = out.scratch_slot[f];
+            if (slot < 0)
+              continue;
+#pragma unroll
+            for (int k = 0; k < NPC; ++k)
+              lds[k * cb_threads + tid] = acc[f][k];
+            if (tid == 0)
+              lds[zero_at] = number(0);
+            __syncthreads();
+            for (int col = tid; col < n_cols; col += cb_threads)
+              {
+                const GatherAxis t0 = gather_table[0][col % ox];
+                const GatherAxis t1 = gather_table[1][col / ox];
+                if (t0.goff < 0 || t1.goff < 0)
+                  continue;
+                int           xy[4];
+                unsigned int  pres[4];
+#pragma unroll
+                for (int a1 = 0; a1 < 2; ++a1)
+#pragma unroll
+                  for (int a0 = 0; a0 < 2; ++a0)
+                    {
+                      xy[2 * a1 + a0]   = t0.a[a0] + t1.a[a1];
+                      pres[2 * a1 + a0] = (t0.slots >> a0) & (t1.slots >> a1) & 1u;
+                    }
+                const int64_t goff_xy = t0.goff + t1.goff;
+                const double  mass_xy = t0.mass * t1.mass;
+                const unsigned int faces_xy = t0.faces | t1.faces;
+                number &kept = carry[slot * n_cols + col];
+#pragma unroll
+                for (int lz = 0; lz <= degree * cc_ez; ++lz)
+                  {
+                    const int P = degree * zs + lz;
+                    const bool top = lz == degree * cc_ez;
+                    const bool owned = (P >= pz0 && P < pz1) || P == pz_close;
+                    if (!owned)
+                      continue;
+                    // z slots: lower cell (or the kept sum), upper cell
+                    number sum = number(0);
+                    if (degree == 2 && (lz & 1))
+                      {
+                        const int o = ((lz - 1) / 2) * cz + kz;
+#pragma unroll
+                        for (int s = 0; s < 4; ++s)
+                          sum += lds[pres[s] ? xy[s] + o : zero_at];
+                      }
+                    else
+                      {
+                        if (lz == 0)
+                          sum = first ? number(0) : kept;
+                        else
+                          {
+                            const int o = (lz / degree - 1) * cz + degree * kz;
+#pragma unroll
+                            for (int s = 0; s < 4; ++s)
+                              sum += lds[pres[s] ? xy[s] + o : zero_at];
+                          }
+                        if (top && P != pz_close)
+                          {
+                            kept = sum; // completed by the next slab
+                            continue;
+                          }
+                        if (!top)
+                          {
+                            const int o = (lz / degree) * cz;
+#pragma unroll
+                            for (int s = 0; s < 4; ++s)
+                              sum += lds[pres[s] ? xy[s] + o : zero_at];
+                          }
+                      }
+                    unsigned int faces = faces_xy;
+                    if (!g.periodic[2])
+                      faces |= (P == 0 ? 1u << 4 : 0u) | (P == g.nodes[2] - 1 ? 1u << 5 : 0u);
+                    cellblock::finish_node(out, f, sum, faces,
+                                           mass_xy * lumped_mass_1d<degree>(g, 2, P),
+                                           goff_xy + int64_t(P) * zstride);
+                  }
+              }
+            __syncthreads(); // contributions of this field no longer read
+          }
+      }
+  }
+
+  /// Shared memory of a cell_block_kernel (dim < 3) or cell_column_kernel (dim 3)
+  /// launch in bytes, for `n_in` staged inputs and `n_out` output fields.
+  template <int dim, int degree, typename number>
+  inline int64_t
+  cell_block_shared_bytes(int n_in, int n_out)
+  {
+    int64_t box = 1;
+    for (int d = 0; d < dim; ++d)
+      box *= int64_t(degree) * (dim == 3 ? (d == 2 ? cc_ez : cc_ex) : cb_extent<dim>(d)) + 1;
+    const int64_t contrib = int64_t(cb_threads) * dofs_per_cell<dim, degree> + 1; // + zero entry
+    int64_t       n       = std::max(int64_t(n_in) * box, contrib);
+    if (dim == 3)
+      n += int64_t(n_out) * (degree * (cc_ex - 1) + 1) * (degree * (cc_ey - 1) + 1);
+    return n * int64_t(sizeof(number));
+  }
+
+  // ------------------------------------------------------------------------------
   // Collocation-split path
   // ------------------------------------------------------------------------------
 

@@ -793,7 +793,7 @@ namespace ppf
                                    dim3(nb), dim3(block_size), 0, stream, pde, g, deps, o,
                                    plan.terms, timer);
             }
-          else
+          else if (!launch_cell_block<number, id>(lhs, g, deps, o))
             {
               number *scratch = ensure_scratch<number>(int64_t(bf.size()) *
                                                        dofs_per_cell<dim, degree> * g.n_cells);
@@ -902,6 +902,159 @@ namespace ppf
             }
         }
       launch(use_split, out);
+    }
+
+    /// Launches the one-pass generic kernel (cell_column_kernel in 3-D,
+    /// cell_block_kernel otherwise); false if it is switched off or its staged inputs
+    /// exceed the shared memory of a workgroup, in which case the two-pass cell/gather
+    /// kernels run instead.
+    template <typename number, int id>
+    bool
+    launch_cell_block(bool lhs, const GridDesc &g, const DepTable<number> &deps,
+                      const OutTable<number> &out)
+    {
+      if (!generic_cell_block)
+        return false;
+      CellBlockArgs cb {};
+      for (int f = 0; f < App::n_fields; ++f)
+        for (int s = 0; s < PPF_N_SLOTS; ++s)
+          if (deps.ptr[f][s] && (deps.flags[f][s] & (slot_values | slot_gradients)))
+            {
+              if (cb.n_in == cb_max_inputs)
+                return false;
+              cb.in_field[cb.n_in] = f;
+              cb.in_slot[cb.n_in]  = s;
+              cb.pos[f][s]         = cb.n_in;
+              ++cb.n_in;
+            }
+      int n_out = 0;
+      for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+        n_out += out.scratch_slot[f] >= 0 ? 1 : 0;
+      const int64_t bytes = cell_block_shared_bytes<dim, degree, number>(cb.n_in, n_out);
+      if (bytes > cell_block_lds_limit())
+        return false;
+      int64_t n_blocks = 1;
+      if constexpr (dim == 3)
+        {
+          cb.nblk[0]  = (g.cells[0] + cc_ex - 2) / (cc_ex - 1);
+          cb.nblk[1]  = (g.cells[1] + cc_ey - 2) / (cc_ey - 1);
+          cb.bcell[0] = cc_ex - 1;
+          cb.bcell[1] = cc_ey - 1;
+          // z chunks: enough columns to fill every compute unit to the kernel's
+          // occupancy, each chunk as long as that allows
+          const void *kernel =
+            lhs ? reinterpret_cast<const void *>(
+                    &cell_column_kernel<PDE<number>, degree, number, true, id>)
+                : reinterpret_cast<const void *>(
+                    &cell_column_kernel<PDE<number>, degree, number, false, id>);
+          int resident = 1;
+          if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kernel, cb_threads,
+                                                           size_t(bytes)) != hipSuccess)
+            {
+              (void)hipGetLastError();
+              resident = 1;
+            }
+          const int64_t target = device_cus() * std::max(1, resident);
+          const int64_t tiles  = int64_t(cb.nblk[0]) * cb.nblk[1];
+          const int64_t chunks = std::max<int64_t>(1, (target + tiles - 1) / tiles);
+          cb.bcell[2]            = column_chunk > 0 ? std::min(column_chunk, g.cells[2])
+                                                    : int((g.cells[2] + chunks - 1) / chunks);
+          cb.nblk[2]             = (g.cells[2] + cb.bcell[2] - 1) / cb.bcell[2];
+          n_blocks               = tiles * cb.nblk[2];
+        }
+      else
+        for (int d = 0; d < 3; ++d)
+          {
+            if (d < dim)
+              {
+                cb.bcell[d] = cb_extent<dim>(d) - 1;
+                cb.nblk[d]  = (g.cells[d] + cb.bcell[d] - 1) / cb.bcell[d];
+              }
+            else
+              {
+                cb.bcell[d] = 1;
+                cb.nblk[d]  = 1;
+              }
+            n_blocks *= cb.nblk[d];
+          }
+      if (n_blocks > int64_t(std::numeric_limits<int>::max()))
+        return false;
+      const auto &pde = *pde_of<number>();
+      auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(unsigned(n_blocks)), dim3(cb_threads),
+                           unsigned(bytes), stream, pde, g, deps, out, timer, cb);
+      };
+      if constexpr (dim == 3)
+        {
+          if (lhs)
+            go(cell_column_kernel<PDE<number>, degree, number, true, id>);
+          else
+            go(cell_column_kernel<PDE<number>, degree, number, false, id>);
+        }
+      else
+        {
+          if (lhs)
+            go(cell_block_kernel<PDE<number>, dim, degree, number, true, id>);
+          else
+            go(cell_block_kernel<PDE<number>, dim, degree, number, false, id>);
+        }
+      dirichlet_faces(g, out);
+      return true;
+    }
+
+    /// Writes the constraint values on the Dirichlet faces of every output of an
+    /// explicit update (the operator kernels leave those nodes).
+    template <typename number>
+    void
+    dirichlet_faces(const GridDesc &g, const OutTable<number> &out)
+    {
+      if (out.dirichlet_write != dirichlet_value)
+        return;
+      int n[3], per[3];
+      for (int d = 0; d < 3; ++d)
+        {
+          n[d]   = d < dim ? g.nodes[d] : 1;
+          per[d] = d < dim ? g.periodic[d] : 1;
+        }
+      int64_t face = 0;
+      for (int d = 0; d < 3; ++d)
+        face = std::max(face, int64_t(n[(d + 1) % 3]) * n[(d + 2) % 3]);
+      for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+        if ((out.out_mask & (1u << f)) && out.dirichlet_mask[f])
+          hipLaunchKernelGGL(dirichlet_faces_kernel<number>,
+                             dim3(unsigned((face + 255) / 256), 6), dim3(256), 0, stream,
+                             out.dst[f], out.dirichlet_values[f], n[0], n[1], n[2], per[0],
+                             per[1], per[2], out.dirichlet_mask[f], out.halt);
+    }
+
+    /// Compute units of the current device.
+    int64_t
+    device_cus()
+    {
+      if (n_cus_ == 0)
+        {
+          int dev = 0, v = 0;
+          PPF_HIP_CHECK(hipGetDevice(&dev));
+          PPF_HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+          n_cus_ = std::max(1, v);
+        }
+      return n_cus_;
+    }
+
+    /// Dynamic shared memory one cell-block launch may use: the device's per-workgroup
+    /// limit less room for the kernels' static tables.
+    int64_t
+    cell_block_lds_limit()
+    {
+      if (lds_limit_ == 0)
+        {
+          int dev = 0, v = 0;
+          PPF_HIP_CHECK(hipGetDevice(&dev));
+          PPF_HIP_CHECK(
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+          lds_limit_ = std::max<int64_t>(int64_t(v) - 4096, 0);
+        }
+      return lds_limit_;
     }
 
     /// Launches the tiled stencil kernel for a split operator; false if the operator
@@ -1179,19 +1332,7 @@ namespace ppf
         go(I4 {}, I2 {}, I1 {});
       else
         go(I4 {}, I4 {}, I1 {});
-      if (out.dirichlet_write == dirichlet_value)
-        for (int f = 0; f < PPF_MAX_FIELDS; ++f)
-          if ((out.out_mask & (1u << f)) && out.dirichlet_mask[f])
-            {
-              int64_t face = 0;
-              for (int d = 0; d < 3; ++d)
-                face = std::max(face, int64_t(a.n[(d + 1) % 3]) * a.n[(d + 2) % 3]);
-              hipLaunchKernelGGL(dirichlet_faces_kernel<number>,
-                                 dim3(unsigned((face + 255) / 256), 6), dim3(256), 0, stream,
-                                 out.dst[f], out.dirichlet_values[f], a.n[0], a.n[1], a.n[2],
-                                 a.periodic[0], a.periodic[1], a.periodic[2],
-                                 out.dirichlet_mask[f], out.halt);
-            }
+      dirichlet_faces(g, out);
       return true;
     }
 
@@ -3047,6 +3188,13 @@ namespace ppf
     int                                    stencil_z_chunk = 0;
     /// Workgroup -> tile order of the stencil kernel (StencilArgs::map_mode).
     int                                    stencil_map     = 0;
+    /// Generic path kernel: the one-pass cell-block kernel (true) or the two-pass
+    /// cell/gather kernels through a global scratch array (false).
+    bool                                   generic_cell_block = true;
+    /// z layers per column of cell_column_kernel (0: enough columns to fill the device).
+    int                                    column_chunk       = 0;
+    int64_t                                n_cus_     = 0;
+    int64_t                                lds_limit_ = 0;
     int                                    stencil_target_blocks = 2048;
     /// Cache policy of the stencil kernel's output stores (Auto: non-temporal for
     /// launches whose arrays exceed stencil_nt_min_bytes).

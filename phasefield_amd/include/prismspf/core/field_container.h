@@ -123,6 +123,8 @@ public:
   {
     const int slot = dependency_slot(type);
     read_value_mask[slot] |= 1u << field_index;
+    if (block_view)
+      return ScalarValue(block_ptr[field_index][slot][block_offset(int(cell_q))]);
     if (cell_view)
       return ScalarValue(cell_values[field_index][slot][cell_q]);
     return ScalarValue(value[field_index][slot]);
@@ -134,6 +136,23 @@ public:
     const int slot = dependency_slot(type);
     read_gradient_mask[slot] |= 1u << field_index;
     Gradient<Scalar> g;
+    if (block_view)
+      {
+        // As the cell view, with the cell's nodes read from the staged node block.
+        using Basis     = GLLBasis<degree>;
+        const number *u = block_ptr[field_index][slot];
+        for (unsigned int d = 0; d < dim; ++d)
+          {
+            const int qd  = local_coord<degree>(int(cell_q), int(d));
+            number    sum = number(0);
+            for (int k = 0; k <= int(degree); ++k)
+              if (Basis::D[qd][k] != 0.0)
+                sum += number(Basis::D[qd][k]) *
+                       u[block_offset(with_coord<degree>(int(cell_q), int(d), k))];
+            g[d] = ScalarValue(sum * inv_h[d]);
+          }
+        return g;
+      }
     if (cell_view)
       {
         // Collocation: the gradient at node q of the cell is the 1-D derivative matrix
@@ -306,6 +325,25 @@ public:
   unsigned int cell_q    = 0;
   number       inv_h[dim];
   number       cell_values[PPF_MAX_FIELDS][PPF_N_SLOTS][dofs_per_component];
+
+  /// Block view: the cell's nodal values are read from a staged block of nodes (a
+  /// workgroup's shared memory): local node k of the cell is at
+  /// block_ptr[f][s][block_base + kx + block_stride[0] ky + block_stride[1] kz].
+  bool          block_view = false;
+  const number *block_ptr[PPF_MAX_FIELDS][PPF_N_SLOTS];
+  int           block_base      = 0;
+  int           block_stride[2] = {0, 0};
+
+  [[nodiscard]] PPF_HD PPF_INLINE int
+  block_offset(int k) const
+  {
+    int o = block_base + local_coord<degree>(k, 0);
+    if constexpr (dim > 1)
+      o += block_stride[0] * local_coord<degree>(k, 1);
+    if constexpr (dim > 2)
+      o += block_stride[1] * local_coord<degree>(k, 2);
+    return o;
+  }
 
   number value_term[PPF_MAX_FIELDS];
   number gradient_term[PPF_MAX_FIELDS][dim];

@@ -61,17 +61,22 @@ OPERATOR_CASES = [
 
 
 @pytest.mark.parametrize("app,text,ops", OPERATOR_CASES, ids=[c[0] for c in OPERATOR_CASES])
-@pytest.mark.parametrize("path", ["auto", "generic", "split_node"])
+@pytest.mark.parametrize("path", ["auto", "generic", "generic_two_pass", "split_node"])
 def test_operator_matches_oracle(app, text, ops, path):
     """auto: tiled stencil kernel for split hooks, generic cell path otherwise;
-    generic: cell path for every hook; split_node: one-thread-per-node split kernel
-    (hooks without the split structure are rejected with an error)."""
+    generic: cell path for every hook (one-pass cell-block kernel); generic_two_pass: the
+    cell path's two-pass kernels (per-cell results through a global array, then a node
+    gather); split_node: one-thread-per-node split kernel (hooks without the split
+    structure are rejected with an error)."""
     ctx, orc = Context(app, text), Oracle(app, text)
     info = ctx.info
     rng = np.random.default_rng(7)
     seed_fields(ctx, orc, rng)
     ctx.set_time(3, 0.37)
     orc.set_time(3, 0.37)
+    if path == "generic_two_pass":
+        ctx.set_option("generic_kernel", "two_pass")
+        path = "generic"
     ctx.set_option("operator_path", path)
     for block_id, which, nb in ops:
         for scaling in (SCALE_NONE, SCALE_INVM, SCALE_INVM_SQRT):
@@ -93,6 +98,74 @@ def test_operator_matches_oracle(app, text, ops, path):
             err = rel_err(got, block_to_stored(ref, info, nb))
             print(app, path, block_id, which, scaling, err, flush=True)
             assert err <= 1e-12, (block_id, which, scaling, err)
+
+
+# Lattices for the one-pass generic kernels: (2-D) blocks of 16 x 16 evaluated cells,
+# 15 x 15 owned; (3-D) columns of 9 x 9 cells, 8 x 8 owned, marching over chunks of z
+# layers in slabs of 3 (chunk 0: automatic; otherwise the given number of layers).
+CELL_BLOCK_CASES = [
+    # 128 x 16 x 16 Q2 cells, Dirichlet on every face: 32 columns, short chunks.
+    ("poisson_3d_q2", prm.poisson(dim=3, refinement=4, mg_depth=1, subdivisions=[8, 1, 1]),
+     [(1, OP_LHS, 1), (1, OP_LHS_PLAIN, 1), (1, OP_RHS, 1)], 0),
+    # The same lattice turned: 16 x 16 x 128 cells, z chunks of 7 layers (3 slabs each,
+    # the last one past the chunk), kept sums carried between slabs.
+    ("poisson_3d_q2", prm.poisson(dim=3, refinement=4, mg_depth=1, subdivisions=[1, 1, 8]),
+     [(1, OP_LHS, 1), (1, OP_LHS_PLAIN, 1)], 7),
+    # Periodic, 2 cells per direction: fewer cells than a column, wrapped halo.
+    ("cahn_hilliard_3d_q2", prm.cahn_hilliard(size=4.0, refinement=1),
+     [(0, OP_RHS, 1), (1, OP_RHS, 1), (2, OP_RHS, 1)], 0),
+    # Periodic, 16^3 Q2 cells in one chunk per column (6 slabs), wrapped first layer.
+    ("cahn_hilliard_3d_q2", prm.cahn_hilliard(size=32.0, refinement=4),
+     [(0, OP_RHS, 1), (1, OP_RHS, 1)], 16),
+    # Periodic, 160 x 96 Q1 cells: ragged blocks along both axes.
+    ("allen_cahn_2d_q1", prm.allen_cahn(dim=2, size=30.0, refinement=5, subdivisions=[5, 3]),
+     [(1, OP_RHS, 1), (2, OP_RHS, 1)], 0),
+    # Non-separable hooks, mixed Dirichlet/natural faces, 16^3 Q1 cells; chunks of 5.
+    ("testbed_3d_q1", prm.testbed(dim=3, refinement=4, dirichlet_faces=(1, 4)),
+     [(0, OP_RHS, 1), (1, OP_LHS, 1), (1, OP_LHS_PLAIN, 1), (2, OP_RHS, 1)], 5),
+    ("testbed_3d_q1", prm.testbed(dim=3, refinement=4, dirichlet_faces=(0, 5)),
+     [(0, OP_RHS, 1), (1, OP_LHS, 1), (2, OP_RHS, 1)], 0),
+    ("testbed_2d_q2", prm.testbed(dim=2, refinement=5),
+     [(0, OP_RHS, 1), (1, OP_RHS, 1), (1, OP_LHS, 1), (2, OP_RHS, 1)], 0),
+    # Three coupled component fields (linear elasticity), 8^3 and 16^3 Q2 cells.
+    ("mechanics_3d_q2", prm.mechanics(refinement=3), [(1, OP_LHS, 3), (1, OP_RHS, 3)], 0),
+    ("mechanics_3d_q2", prm.mechanics(refinement=4), [(1, OP_LHS, 3)], 16),
+]
+
+
+@pytest.mark.parametrize("app,text,ops,chunk", CELL_BLOCK_CASES,
+                         ids=[f"{c[0]}-{i}" for i, c in enumerate(CELL_BLOCK_CASES)])
+def test_cell_block_kernel_matches_oracle_and_two_pass(app, text, ops, chunk):
+    """The one-pass generic kernel against the oracle (1e-12) and against the two-pass
+    kernels, which sum the same per-cell contributions in the same order (1e-14)."""
+    ctx, orc = Context(app, text), Oracle(app, text)
+    info = ctx.info
+    rng = np.random.default_rng(17)
+    seed_fields(ctx, orc, rng)
+    ctx.set_time(2, 0.21)
+    orc.set_time(2, 0.21)
+    ctx.set_option("operator_path", "generic")
+    ctx.set_option("generic_column_chunk", chunk)
+    for block_id, which, nb in ops:
+        for scaling in (SCALE_NONE, SCALE_INVM):
+            src_stored = rng.uniform(-1, 1, nb * info.n_nodes)
+            ref = block_to_stored(orc.apply_operator(block_id, which, scaling,
+                                                     block_to_full(src_stored, info, nb), nb),
+                                  info, nb)
+            d_src = dev(src_stored)
+            got = {}
+            for kernel in ("cell_block", "two_pass"):
+                ctx.set_option("generic_kernel", kernel)
+                d_dst = torch.full_like(d_src, float("nan"))
+                ptr = d_src.data_ptr() if which != OP_RHS else 0
+                ctx.apply_operator(block_id, which, scaling, FP64, ptr, d_dst.data_ptr())
+                got[kernel] = d_dst.cpu().numpy()
+            err = rel_err(got["cell_block"], ref)
+            diff = rel_err(got["cell_block"], got["two_pass"])
+            print(app, block_id, which, scaling, err, diff,
+                  np.array_equal(got["cell_block"], got["two_pass"]), flush=True)
+            assert err <= 1e-12, (block_id, which, scaling, err)
+            assert diff <= 1e-14, (block_id, which, scaling, diff)
 
 
 # Tile shapes of the stencil kernel: (waves per workgroup, rows per wave, z-planes per

@@ -3,10 +3,11 @@ for callers that cannot allocate) against ppf_apply_operator on the same inputs:
 same kernels run, so results agree bit for bit.  Each case makes two calls in a row, the
 second on a smaller lattice (less scratch), so nothing left in the shared device array
 from the first call may leak into the second.  Covered: the tiled stencil path
-(Cahn-Hilliard), the generic cell path whose per-cell results live in the scratch
-(linear elasticity, FP64; test bed, FP32 with FP32 copies of the non-src fields), an
-operator whose scratch need exceeds the array (a clear error), and the free device
-memory before and after a first call on a fresh context (unchanged)."""
+(Cahn-Hilliard), the generic path's one-pass cell-block kernel (linear elasticity, FP64;
+no scratch) and its two-pass kernels whose per-cell results live in the scratch (linear
+elasticity, FP64; test bed, FP32 with FP32 copies of the non-src fields), an operator
+whose scratch need exceeds the array (a clear error), and the free device memory before
+and after a first call on a fresh context (unchanged)."""
 
 from __future__ import annotations
 
@@ -48,25 +49,29 @@ def both(ctx, block, which, scaling, prec, nb, rng):
 
 CASES = [
     # (app, [prm text of the first, larger lattice; of the second, smaller one],
-    #  block, which, scaling, precision, fields in the block, operator path)
+    #  block, which, scaling, precision, fields in the block, operator path,
+    #  generic-path kernel)
     ("cahn_hilliard_3d_q2", [prm.cahn_hilliard(size=16.0, refinement=4),
                              prm.cahn_hilliard(size=8.0, refinement=3)],
-     0, OP_RHS, SCALE_INVM, FP64, 1, "auto"),
+     0, OP_RHS, SCALE_INVM, FP64, 1, "auto", "cell_block"),
     ("mechanics_3d_q2", [prm.mechanics(refinement=3), prm.mechanics(refinement=2)],
-     1, OP_LHS, SCALE_NONE, FP64, 3, "auto"),
+     1, OP_LHS, SCALE_NONE, FP64, 3, "auto", "cell_block"),
+    ("mechanics_3d_q2", [prm.mechanics(refinement=3), prm.mechanics(refinement=2)],
+     1, OP_LHS, SCALE_NONE, FP64, 3, "auto", "two_pass"),
     ("testbed_2d_q2", [prm.testbed(dim=2, refinement=5), prm.testbed(dim=2, refinement=3)],
-     1, OP_LHS, SCALE_NONE, FP32, 1, "generic"),
+     1, OP_LHS, SCALE_NONE, FP32, 1, "generic", "two_pass"),
 ]
 
 
-@pytest.mark.parametrize("app,texts,block,which,scaling,prec,nb,path", CASES,
-                         ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("app,texts,block,which,scaling,prec,nb,path,kernel", CASES,
+                         ids=[f"{c[0]}-{c[-1]}" for c in CASES])
 def test_noalloc_matches_operator_two_calls(device, app, texts, block, which, scaling, prec,
-                                            nb, path):
+                                            nb, path, kernel):
     rng = np.random.default_rng(5)
     for text in texts:  # the second call runs on a smaller lattice
         ctx = Context(app, text)
         ctx.set_option("operator_path", path)
+        ctx.set_option("generic_kernel", kernel)
         seeded(ctx, rng)
         print(app, ctx.info.n_nodes, flush=True)
         a, b = both(ctx, block, which, scaling, prec, nb, rng)
@@ -76,8 +81,10 @@ def test_noalloc_matches_operator_two_calls(device, app, texts, block, which, sc
 
 
 def test_noalloc_rejects_oversized_scratch(device):
-    # 64^3 Q2 elasticity cells on the generic path: 3 x 27 x 8 B per cell = 170 MB.
+    # 64^3 Q2 elasticity cells on the two-pass generic kernels: 3 x 27 x 8 B per cell =
+    # 170 MB of scratch.
     ctx = Context("mechanics_3d_q2", prm.mechanics(refinement=6))
+    ctx.set_option("generic_kernel", "two_pass")
     n = 3 * ctx.info.n_nodes
     src = torch.zeros(n, dtype=torch.float64, device="cuda")
     dst = torch.zeros_like(src)
@@ -90,12 +97,14 @@ def test_noalloc_rejects_oversized_scratch(device):
 
 
 def test_noalloc_first_call_allocates_nothing(device):
-    text = prm.mechanics(refinement=4)  # generic path: 4096 cells, 2.7 MB of scratch
+    text = prm.mechanics(refinement=4)  # two-pass kernels: 4096 cells, 2.7 MB of scratch
     warm = Context("mechanics_3d_q2", prm.mechanics(refinement=2))
+    warm.set_option("generic_kernel", "two_pass")
     rng = np.random.default_rng(9)
     both(warm, 1, OP_LHS, SCALE_NONE, FP64, 3, rng)  # kernels loaded
     warm.close()
     ctx = Context("mechanics_3d_q2", text)
+    ctx.set_option("generic_kernel", "two_pass")
     n = 3 * ctx.info.n_nodes
     src = torch.tensor(rng.uniform(-1, 1, n), device="cuda")
     dst = torch.zeros_like(src)
@@ -109,4 +118,15 @@ def test_noalloc_first_call_allocates_nothing(device):
     ctx.apply_operator(1, OP_LHS, SCALE_NONE, FP64, src.data_ptr(), dst.data_ptr())
     ctx.synchronize()
     assert torch.cuda.mem_get_info()[0] < free1
+    ctx.close()
+
+
+def test_noalloc_cell_block_needs_no_scratch(device):
+    # The one-pass generic kernel keeps its per-cell results in shared memory: the
+    # 64^3 elasticity operator that overflows the array on the two-pass kernels runs.
+    rng = np.random.default_rng(11)
+    ctx = Context("mechanics_3d_q2", prm.mechanics(refinement=6))
+    a, b = both(ctx, 1, OP_LHS, SCALE_NONE, FP64, 3, rng)
+    assert np.isfinite(a).all()
+    assert np.array_equal(a, b), float(np.abs(a - b).max())
     ctx.close()
