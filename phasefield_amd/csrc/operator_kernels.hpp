@@ -448,6 +448,8 @@ namespace ppf
     int bcell[3]; ///< owned cells per block and axis (column kernel: 8, 8, chunk layers)
     /// Staged position of each (field, slot), 0 for those not staged (never read).
     int pos[PPF_MAX_FIELDS][PPF_N_SLOTS];
+    /// cell_column_kernel: a second contribution buffer, output fields gathered in pairs.
+    int pair;
   };
 
   namespace cellblock
@@ -656,26 +658,41 @@ namespace ppf
       return c;
     }
 
-    /// Writes an owned node: the scaled sum, zero at a Dirichlet node, nothing at a
-    /// Dirichlet node of an explicit update (dirichlet_faces_kernel writes those
-    /// values afterwards; the kernels read no global memory while storing, so no store
-    /// waits for an earlier one).
+    /// What the write-out of one output field needs, read from the kernel arguments
+    /// once per field (a load from the arguments inside the node loop would also wait
+    /// for every shared-memory read in flight).
     template <typename number>
-    PPF_HD PPF_INLINE void
-    finish_node(const OutTable<number> &out, int f, number sum, unsigned int faces,
-                double mass, int64_t node)
+    struct FieldOut
     {
-      if (faces & out.dirichlet_mask[f])
-        {
-          if (out.dirichlet_write != dirichlet_value)
-            out.dst[f][node] = number(0);
-          return;
-        }
-      double scale = 1.0;
-      if (out.scaling != scale_none)
-        scale = out.scaling == scale_invm ? 1.0 / mass : 1.0 / sqrt(mass);
-      out.dst[f][node] = number(scale * double(sum));
-    }
+      number      *dst;
+      unsigned int mask;  ///< Dirichlet faces
+      bool         keep;  ///< Dirichlet nodes are left to dirichlet_faces_kernel
+      int          scaling;
+
+      PPF_HD PPF_INLINE
+      FieldOut(const OutTable<number> &out, int f)
+        : dst(out.dst[f])
+        , mask(out.dirichlet_mask[f])
+        , keep(out.dirichlet_write == dirichlet_value)
+        , scaling(out.scaling)
+      {}
+
+      /// Writes an owned node: the scaled sum, zero at a Dirichlet node, nothing at a
+      /// Dirichlet node of an explicit update (dirichlet_faces_kernel writes those
+      /// values afterwards; the kernels read no global memory while storing, so no
+      /// store waits for an earlier one).
+      PPF_HD PPF_INLINE void
+      write(number sum, unsigned int faces, double mass, int64_t node) const
+      {
+        const bool dir   = (faces & mask) != 0;
+        double     scale = 1.0;
+        if (scaling != scale_none)
+          scale = scaling == scale_invm ? 1.0 / mass : 1.0 / sqrt(mass);
+        const number v = dir ? number(0) : number(scale * double(sum));
+        if (!(dir && keep))
+          dst[node] = v;
+      }
+    };
   } // namespace cellblock
 
   template <class PDE, int dim, int degree, typename number, bool lhs, int block_id>
@@ -796,6 +813,7 @@ namespace ppf
         if (tid == 0)
           lds[zero_at] = number(0);
         __syncthreads();
+        const cellblock::FieldOut<number> fo(out, f);
         for (int c = 0; c < n_cand; ++c)
           {
             const int        i  = tid + c * cb_threads;
@@ -817,8 +835,8 @@ namespace ppf
                                                  (t2.slots >> a2) & 1u;
                     sum += lds[present ? t0.a[a0] + t1.a[a1] + t2.a[a2] : zero_at];
                   }
-            cellblock::finish_node(out, f, sum, t0.faces | t1.faces | t2.faces,
-                                   t0.mass * t1.mass * t2.mass, t0.goff + t1.goff + t2.goff);
+            fo.write(sum, t0.faces | t1.faces | t2.faces, t0.mass * t1.mass * t2.mass,
+                     t0.goff + t1.goff + t2.goff);
           }
         __syncthreads(); // contributions of this field no longer read
       }
@@ -851,8 +869,10 @@ namespace ppf
     constexpr int cz      = cc_ex * cc_ey;         // cell stride along z
     extern __shared__ unsigned char cb_shared[];
     number *lds = reinterpret_cast<number *>(cb_shared);
+    // buffer A: the staged box, then contributions; buffer B (cb.pair): contributions
     const int region = (cb.n_in * box_n > zero_at + 1 ? cb.n_in * box_n : zero_at + 1);
-    number   *carry  = lds + region; // [output slot][column]
+    number   *lds_b  = lds + region;
+    number   *carry  = lds_b + (cb.pair ? zero_at + 1 : 0); // [output slot][column]
 
     if (halted(out.halt))
       return;
@@ -924,90 +944,110 @@ namespace ppf
         }
         __syncthreads(); // every cell evaluated: the staged inputs are no longer read
 
-        // 3. Per output field: contributions, then every owned column of the tile walks
-        //    the slab's node planes.
+        // 3. Per output field: contributions to a shared-memory buffer, then every owned
+        //    column of the tile walks the slab's node planes.  With a second buffer
+        //    (cb.pair) the fields go two at a time: fewer barriers, two independent
+        //    gathers per column, and the integrated contributions leave registers early.
+        const auto gather = [&](const cellblock::FieldOut<number> &fo, number *kept_base,
+                                const number *buf, int col, const GatherAxis &t0,
+                                const GatherAxis &t1) {
+          int          xy[4];
+          unsigned int pres[4];
+#pragma unroll
+          for (int a1 = 0; a1 < 2; ++a1)
+#pragma unroll
+            for (int a0 = 0; a0 < 2; ++a0)
+              {
+                xy[2 * a1 + a0]   = t0.a[a0] + t1.a[a1];
+                pres[2 * a1 + a0] = (t0.slots >> a0) & (t1.slots >> a1) & 1u;
+              }
+          number &kept = kept_base[col];
+          // The sums of all planes first (every read issued without a branch), each in
+          // node_gather_kernel's order: lower cell (or the kept sum), then upper cell.
+          constexpr int NZ = degree * cc_ez + 1;
+          number        part[NZ];
+#pragma unroll
+          for (int lz = 0; lz < NZ; ++lz)
+            {
+              number sum = number(0);
+              if (lz == 0)
+                sum = first ? number(0) : kept;
+              else
+                {
+                  const int o = (degree == 2 && (lz & 1)) ? ((lz - 1) / 2) * cz + kz
+                                                          : (lz / degree - 1) * cz + degree * kz;
+#pragma unroll
+                  for (int s = 0; s < 4; ++s)
+                    sum += buf[pres[s] ? xy[s] + o : zero_at];
+                }
+              if (lz < NZ - 1 && !(degree == 2 && (lz & 1)))
+                {
+                  const int o = (lz / degree) * cz;
+#pragma unroll
+                  for (int s = 0; s < 4; ++s)
+                    sum += buf[pres[s] ? xy[s] + o : zero_at];
+                }
+              part[lz] = sum;
+            }
+          const int64_t      goff_xy  = t0.goff + t1.goff;
+          const double       mass_xy  = t0.mass * t1.mass;
+          const unsigned int faces_xy = t0.faces | t1.faces;
+#pragma unroll
+          for (int lz = 0; lz < NZ; ++lz)
+            {
+              const int  P     = degree * zs + lz;
+              const bool owned = (P >= pz0 && P < pz1) || P == pz_close;
+              if (!owned)
+                continue;
+              if (lz == NZ - 1 && P != pz_close)
+                {
+                  kept = part[lz]; // completed by the next slab
+                  continue;
+                }
+              unsigned int faces = faces_xy;
+              if (!g.periodic[2])
+                faces |= (P == 0 ? 1u << 4 : 0u) | (P == g.nodes[2] - 1 ? 1u << 5 : 0u);
+              fo.write(part[lz], faces, mass_xy * lumped_mass_1d<degree>(g, 2, P),
+                       goff_xy + int64_t(P) * zstride);
+            }
+        };
+        // fa: field whose contributions wait in buffer A (-1: none)
+        int fa = -1;
 #pragma unroll
         for (int f = 0; f < PPF_MAX_FIELDS; ++f)
           {
-            const int slot = out.scratch_slot[f];
-            if (slot < 0)
+            if (out.scratch_slot[f] < 0)
               continue;
+            const bool to_b = fa >= 0; // A is taken: this field goes to B, gather both
+            number    *buf  = to_b ? lds_b : lds;
 #pragma unroll
             for (int k = 0; k < NPC; ++k)
-              lds[k * cb_threads + tid] = acc[f][k];
+              buf[k * cb_threads + tid] = acc[f][k];
             if (tid == 0)
-              lds[zero_at] = number(0);
+              buf[zero_at] = number(0);
+            const bool more = (out.out_mask >> (f + 1)) != 0u;
+            if (!to_b && cb.pair && more)
+              {
+                fa = f; // gathered together with the next output field
+                continue;
+              }
             __syncthreads();
+            const cellblock::FieldOut<number> fo(out, f);
+            const cellblock::FieldOut<number> fo_a(out, to_b ? fa : f);
+            number *const kept   = carry + out.scratch_slot[f] * n_cols;
+            number *const kept_a = carry + out.scratch_slot[to_b ? fa : f] * n_cols;
             for (int col = tid; col < n_cols; col += cb_threads)
               {
                 const GatherAxis t0 = gather_table[0][col % ox];
                 const GatherAxis t1 = gather_table[1][col / ox];
                 if (t0.goff < 0 || t1.goff < 0)
                   continue;
-                int           xy[4];
-                unsigned int  pres[4];
-#pragma unroll
-                for (int a1 = 0; a1 < 2; ++a1)
-#pragma unroll
-                  for (int a0 = 0; a0 < 2; ++a0)
-                    {
-                      xy[2 * a1 + a0]   = t0.a[a0] + t1.a[a1];
-                      pres[2 * a1 + a0] = (t0.slots >> a0) & (t1.slots >> a1) & 1u;
-                    }
-                const int64_t goff_xy = t0.goff + t1.goff;
-                const double  mass_xy = t0.mass * t1.mass;
-                const unsigned int faces_xy = t0.faces | t1.faces;
-                number &kept = carry[slot * n_cols + col];
-#pragma unroll
-                for (int lz = 0; lz <= degree * cc_ez; ++lz)
-                  {
-                    const int P = degree * zs + lz;
-                    const bool top = lz == degree * cc_ez;
-                    const bool owned = (P >= pz0 && P < pz1) || P == pz_close;
-                    if (!owned)
-                      continue;
-                    // z slots: lower cell (or the kept sum), upper cell
-                    number sum = number(0);
-                    if (degree == 2 && (lz & 1))
-                      {
-                        const int o = ((lz - 1) / 2) * cz + kz;
-#pragma unroll
-                        for (int s = 0; s < 4; ++s)
-                          sum += lds[pres[s] ? xy[s] + o : zero_at];
-                      }
-                    else
-                      {
-                        if (lz == 0)
-                          sum = first ? number(0) : kept;
-                        else
-                          {
-                            const int o = (lz / degree - 1) * cz + degree * kz;
-#pragma unroll
-                            for (int s = 0; s < 4; ++s)
-                              sum += lds[pres[s] ? xy[s] + o : zero_at];
-                          }
-                        if (top && P != pz_close)
-                          {
-                            kept = sum; // completed by the next slab
-                            continue;
-                          }
-                        if (!top)
-                          {
-                            const int o = (lz / degree) * cz;
-#pragma unroll
-                            for (int s = 0; s < 4; ++s)
-                              sum += lds[pres[s] ? xy[s] + o : zero_at];
-                          }
-                      }
-                    unsigned int faces = faces_xy;
-                    if (!g.periodic[2])
-                      faces |= (P == 0 ? 1u << 4 : 0u) | (P == g.nodes[2] - 1 ? 1u << 5 : 0u);
-                    cellblock::finish_node(out, f, sum, faces,
-                                           mass_xy * lumped_mass_1d<degree>(g, 2, P),
-                                           goff_xy + int64_t(P) * zstride);
-                  }
+                if (to_b)
+                  gather(fo_a, kept_a, lds, col, t0, t1);
+                gather(fo, kept, buf, col, t0, t1);
               }
-            __syncthreads(); // contributions of this field no longer read
+            fa = -1;
+            __syncthreads(); // the buffers are free again
           }
       }
   }
@@ -1016,7 +1056,7 @@ namespace ppf
   /// launch in bytes, for `n_in` staged inputs and `n_out` output fields.
   template <int dim, int degree, typename number>
   inline int64_t
-  cell_block_shared_bytes(int n_in, int n_out)
+  cell_block_shared_bytes(int n_in, int n_out, bool pair)
   {
     int64_t box = 1;
     for (int d = 0; d < dim; ++d)
@@ -1024,7 +1064,8 @@ namespace ppf
     const int64_t contrib = int64_t(cb_threads) * dofs_per_cell<dim, degree> + 1; // + zero entry
     int64_t       n       = std::max(int64_t(n_in) * box, contrib);
     if (dim == 3)
-      n += int64_t(n_out) * (degree * (cc_ex - 1) + 1) * (degree * (cc_ey - 1) + 1);
+      n += (pair ? contrib : 0) +
+           int64_t(n_out) * (degree * (cc_ex - 1) + 1) * (degree * (cc_ey - 1) + 1);
     return n * int64_t(sizeof(number));
   }
 

@@ -930,9 +930,16 @@ namespace ppf
       int n_out = 0;
       for (int f = 0; f < PPF_MAX_FIELDS; ++f)
         n_out += out.scratch_slot[f] >= 0 ? 1 : 0;
-      const int64_t bytes = cell_block_shared_bytes<dim, degree, number>(cb.n_in, n_out);
+      int64_t bytes = cell_block_shared_bytes<dim, degree, number>(cb.n_in, n_out, false);
       if (bytes > cell_block_lds_limit())
         return false;
+      if (dim == 3 && n_out > 1 &&
+          cell_block_shared_bytes<dim, degree, number>(cb.n_in, n_out, true) <=
+            cell_block_lds_limit())
+        {
+          cb.pair = 1;
+          bytes   = cell_block_shared_bytes<dim, degree, number>(cb.n_in, n_out, true);
+        }
       int64_t n_blocks = 1;
       if constexpr (dim == 3)
         {
