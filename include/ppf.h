@@ -101,14 +101,17 @@ int ppf_field_name(ppf_ctx *ctx, int field, char *buf, int len);
  * kernel, split_node: one thread per node); "stencil_waves" = 0|4|8,
  * "stencil_rows" = 0|1|2|4 (waves x rows: 4x2, 4x4, 8x1 or 8x2) and
  * "stencil_z_chunk" = <planes> (0: automatic) tune the tiled kernel's workgroup
- * shape; "stencil_store_policy" = auto|temporal|nontemporal sets the cache policy of
+ * shape; "stencil_min_nodes" = <n> (default 100000): smaller lattices run split
+ * operators with the one-thread-per-node kernel; "stencil_store_policy" = auto|temporal|nontemporal sets the cache policy of
  * its output stores (auto: non-temporal when the launch's arrays exceed 192 MiB);
  * "generic_kernel" = cell_block|two_pass selects the generic path's kernels (cell_block:
  * one launch, cell blocks staged in shared memory, no device scratch; two_pass: a cell
  * kernel into a device scratch array and a node gather; cell_block falls back to
  * two_pass when a block's staged inputs exceed the shared memory of a workgroup);
  * "generic_column_chunk" = <layers> sets the z layers each 3-D cell column marches
- * over (0: automatic, enough columns for every compute unit);
+ * over (0: automatic, enough columns for every compute unit); "mg_graphs" = 1|0 (1:
+ * multigrid V-cycles that need no host synchronization are captured into a HIP graph
+ * and replayed);
  * "synchronize" = 1|0 (0 leaves ppf_apply_operator asynchronous on the
  * context's stream); "write_output" = 0|1 (1: at every output increment ppf_run and
  * ppf_solve_increment write the solution file of the `output` parameters,

@@ -371,6 +371,14 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
       }
     else if (k == "generic_column_chunk")
       c.rt.column_chunk = std::max(0, std::stoi(v));
+    else if (k == "mg_graphs")
+      {
+        if (v != "0" && v != "1")
+          throw ppf::Error(ppf::status_invalid, "mg_graphs: 0 or 1");
+        c.rt.mg_graphs = v == "1";
+      }
+    else if (k == "stencil_min_nodes")
+      c.rt.stencil_min_nodes = std::max<int64_t>(0, std::stoll(v));
     else if (k == "stencil_target_blocks")
       c.rt.stencil_target_blocks = std::max(1, std::stoi(v));
     else if (k == "stencil_store_policy")
@@ -430,6 +438,10 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
       }
     else
       throw ppf::Error(ppf::status_invalid, "unknown option " + k);
+    // captured V-cycles carry the kernels' launch settings
+    if (k.rfind("stencil_", 0) == 0 || k.rfind("generic_", 0) == 0 || k == "operator_path" ||
+        k == "mg_coarse_solver" || k == "mg_level_precision" || k == "mg_graphs")
+      ++c.rt.graph_epoch;
   });
 }
 

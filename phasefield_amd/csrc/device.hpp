@@ -1,6 +1,8 @@
 // Device memory, launch and error helpers of the host runtime.
 #pragma once
 
+#include <atomic>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -39,6 +41,15 @@ namespace ppf
   while (0)
 
   /// Owning device allocation of `n` elements of T.
+  /// Device allocations made by DeviceBuffer so far (a captured launch sequence is
+  /// valid only while no buffer it uses has been reallocated).
+  inline std::atomic<long> &
+  device_allocations()
+  {
+    static std::atomic<long> n {0};
+    return n;
+  }
+
   template <typename T>
   class DeviceBuffer
   {
@@ -82,7 +93,10 @@ namespace ppf
         return;
       release();
       if (n > 0)
-        PPF_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&ptr_), sizeof(T) * size_t(n)));
+        {
+          PPF_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&ptr_), sizeof(T) * size_t(n)));
+          ++device_allocations();
+        }
       n_ = n;
     }
 

@@ -197,6 +197,10 @@ namespace ppf
 
     ~Runtime()
     {
+      for (auto &entry : linear_states)
+        for (auto &e : entry.second.vg_exec)
+          if (e)
+            (void) hipGraphExecDestroy(e);
       if (stream)
         (void) hipStreamDestroy(stream);
     }
@@ -513,6 +517,7 @@ namespace ppf
     void
     sync()
     {
+      ++sync_count_;
       PPF_HIP_CHECK(hipStreamSynchronize(stream));
     }
 
@@ -628,6 +633,7 @@ namespace ppf
       HookProbe<PDE<number>, dim, degree, number> probe(pde, blocks[b], lhs);
       PlanEntry entry;
       entry.plan      = probe.analyse(timer.get_time(), timer.get_timestep(), timer.get_increment());
+      ++graph_epoch;
       entry.time      = timer.get_time();
       entry.dt        = timer.get_timestep();
       entry.increment = timer.get_increment();
@@ -1071,6 +1077,12 @@ namespace ppf
     launch_stencil(int b, bool lhs, const GridDesc &g, const DepTable<number> &deps,
                    const OutTable<number> &out, const SplitPlan &plan)
     {
+      // Small lattices (multigrid's coarse levels) go to the one-thread-per-node kernel:
+      // a tile's march over its z planes would take longer than the whole lattice node
+      // by node.  The fused coarse solve rides on this launch and keeps it.
+      if (g.n_nodes < stencil_min_nodes && !coarse_pending &&
+          !(probe_req.pending && probe_req.block == b))
+        return false;
       StencilArgs<number> a {};
       // Distinct (field, slot) operands of the gradient terms.
       int op_field[stencil_max_ops], op_slot[stencil_max_ops];
@@ -1574,6 +1586,10 @@ namespace ppf
       DeviceBuffer<float>                 vf[6];
       DeviceBuffer<double>                inv_diag_d;
       DeviceBuffer<float>                 inv_diag_f;
+      /// inverse diagonal by node class (diag_table_build_kernel) and its check flag
+      DeviceBuffer<double>                table_d;
+      DeviceBuffer<float>                 table_f;
+      DeviceBuffer<int>                   table_ok;
       double                              lambda_max = 0.0, lambda_min = 0.0;
       bool                                eigenvalues_estimated = false;
       std::vector<DeviceBuffer<double>>   field_d[PPF_MAX_FIELDS];
@@ -1588,6 +1604,8 @@ namespace ppf
       int64_t              n           = 0; // block vector length
       DeviceBuffer<double> x_homog, inhom, rhs, g, h, d, ad, tmp;
       DeviceBuffer<double> inv_diag; // Chebyshev preconditioner on the fine level
+      DeviceBuffer<double> inv_table;
+      DeviceBuffer<int>    inv_table_ok;
       double               cheb_lmax = 0.0, cheb_lmin = 0.0;
       bool                 cheb_eigenvalues_estimated = false;
       DeviceBuffer<double> cheb_d, cheb_ax;
@@ -1597,6 +1615,16 @@ namespace ppf
       std::vector<Level>   levels; // coarsest first
       SolveStats           stats;
       NewtonStats          newton;
+      /// Captured V-cycles (run_vcycle): two slots, keyed by the halt flag the launches
+      /// carry, the capture epoch, the allocation count and the top level's precision.
+      hipGraphExec_t vg_exec[2]  = {nullptr, nullptr};
+      const int     *vg_halt[2]  = {nullptr, nullptr};
+      long           vg_epoch[2] = {-1, -1};
+      long           vg_alloc[2] = {-1, -1};
+      int            vg_prec[2]  = {-1, -1};
+      int            vg_next     = 0;
+      long           vg_warm     = -1;  ///< epoch of the last eager V-cycle found capturable
+      bool           vg_off      = false; ///< the V-cycle synchronizes with the host
     };
 
     const prismspf::LinearSolverParameters &
@@ -2299,6 +2327,8 @@ namespace ppf
           ls.cheb_d.resize(ls.n);
           ls.cheb_ax.resize(ls.n);
           inverse_diagonal<double>(b, fine_fields(), linear_scaling(b), ls.inv_diag.get());
+          build_diag_table<double>(ls.inv_diag.get(), fine, int(blocks[b].field_indices.size()),
+                                   ls.inv_table, ls.inv_table_ok);
           if (!ls.cheb_eigenvalues_estimated)
             estimate_eigenvalues<double>(
               [&](double *dst, const double *src) { vmult_fine(b, dst, src); },
@@ -2318,10 +2348,19 @@ namespace ppf
       if (kind == prismspf::None)
         copy(h, g, ls.n);
       else if (kind == prismspf::Chebyshev)
-        chebyshev<double>(
-          [&](double *dst, const double *src) { vmult_fine(b, dst, src); }, ls.inv_diag.get(),
-          ls.n, lin_params(b).chebyshev_parameters.degree, ls.cheb_lmax, ls.cheb_lmin, h, g,
-          ls.cheb_d.get(), ls.cheb_ax.get(), true);
+        {
+          DiagTable<double> tab;
+          if (ls.inv_table.get() && ls.inv_table_ok.get())
+            {
+              tab.table = ls.inv_table.get();
+              tab.ok    = ls.inv_table_ok.get();
+              tab.grid  = class_grid(fine, int(blocks[b].field_indices.size()));
+            }
+          chebyshev<double>(
+            [&](double *dst, const double *src) { vmult_fine(b, dst, src); },
+            ls.inv_diag.get(), ls.n, lin_params(b).chebyshev_parameters.degree, ls.cheb_lmax,
+            ls.cheb_lmin, h, g, ls.cheb_d.get(), ls.cheb_ax.get(), true, tab);
+        }
       else
         multigrid_vmult(b, h, g);
     }
@@ -2400,10 +2439,70 @@ namespace ppf
     /// Chebyshev iteration of the given degree for A x = rhs preconditioned by the
     /// inverse diagonal.  With `zero_start` x is overwritten (preconditioner use,
     /// deal.II vmult); otherwise x is the starting guess (smoother use, deal.II step).
+    /// Inverse diagonal of a smoother by node class (see diag_table_build_kernel).
+    template <typename T>
+    struct DiagTable
+    {
+      const T   *table = nullptr;
+      const int *ok    = nullptr;
+      ClassGrid  grid {};
+    };
+
+    ClassGrid
+    class_grid(const GridDesc &g, int n_fields) const
+    {
+      ClassGrid c {};
+      for (int d = 0; d < 3; ++d)
+        {
+          c.n[d]        = d < dim ? g.nodes[d] : 1;
+          c.periodic[d] = d < dim ? g.periodic[d] : 1;
+        }
+      c.degree   = degree;
+      c.dim      = dim;
+      c.n_fields = n_fields;
+      c.n_nodes  = g.n_nodes;
+      return c;
+    }
+
+    /// Builds the class table of `inv_diag` into `table` (n_fields * 64 entries) and
+    /// checks every node against it; *ok ends 1 only if all match.
+    template <typename T>
+    DiagTable<T>
+    build_diag_table(const T *inv_diag, const GridDesc &g, int n_fields, DeviceBuffer<T> &table,
+                     DeviceBuffer<int> &ok)
+    {
+      DiagTable<T> t;
+      t.grid = class_grid(g, n_fields);
+      if (int64_t(t.grid.n[0]) * t.grid.n[1] >= (int64_t(1) << 31) || t.grid.n[2] > 65535 ||
+          n_fields > 64)
+        return DiagTable<T> {};
+      if (table.size() < int64_t(n_fields) * diag_classes)
+        table.resize(int64_t(n_fields) * diag_classes);
+      if (ok.size() < 1)
+        ok.resize(1);
+      hipLaunchKernelGGL(diag_table_build_kernel<T>, dim3(1),
+                         dim3(unsigned(n_fields * diag_classes)), 0, stream, table.get(),
+                         ok.get(), inv_diag, t.grid);
+      hipLaunchKernelGGL(diag_table_check_kernel<T>, class_launch(t.grid), dim3(256), 0, stream,
+                         ok.get(), table.get(), inv_diag, t.grid);
+      PPF_HIP_CHECK(hipGetLastError());
+      t.table = table.get();
+      t.ok    = ok.get();
+      return t;
+    }
+
+    static dim3
+    class_launch(const ClassGrid &g)
+    {
+      return dim3(grid_for(int64_t(g.n[0]) * g.n[1], 256), unsigned(g.n[2]),
+                  unsigned(g.n_fields));
+    }
+
     template <typename T, class Vmult>
     void
     chebyshev(Vmult &&A, const T *inv_diag, int64_t n, unsigned int deg, double lmax,
-              double lmin, T *x, const T *rhs, T *d, T *ax, bool zero_start)
+              double lmin, T *x, const T *rhs, T *d, T *ax, bool zero_start,
+              const DiagTable<T> &tab = DiagTable<T> {})
     {
       const double theta = 0.5 * (lmax + lmin);
       const double delta = 0.5 * (lmax - lmin);
@@ -2414,8 +2513,15 @@ namespace ppf
           if (k == 0 && zero_start)
             {
               // From x = 0 the residual is b: no operator application, no clearing.
-              hipLaunchKernelGGL(chebyshev_first_zero_kernel<T>, dim3(grid_for(n, 256)),
-                                 dim3(256), 0, stream, x, d, rhs, inv_diag, T(1.0 / theta), n);
+              if (tab.table)
+                hipLaunchKernelGGL(chebyshev_table_kernel<T>, class_launch(tab.grid),
+                                   dim3(256), 0, stream, x, d, rhs, (const T *)nullptr,
+                                   inv_diag, tab.table, tab.ok, T(0), T(1.0 / theta), 1, 1,
+                                   tab.grid);
+              else
+                hipLaunchKernelGGL(chebyshev_first_zero_kernel<T>, dim3(grid_for(n, 256)),
+                                   dim3(256), 0, stream, x, d, rhs, inv_diag, T(1.0 / theta),
+                                   n);
               continue;
             }
           A(ax, x);
@@ -2429,8 +2535,14 @@ namespace ppf
               c2                   = 2.0 * rho_new / delta;
               rho                  = rho_new;
             }
-          hipLaunchKernelGGL(chebyshev_step_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0,
-                             stream, x, d, rhs, ax, inv_diag, T(c1), T(c2), k == 0 ? 1 : 0, n);
+          if (tab.table)
+            hipLaunchKernelGGL(chebyshev_table_kernel<T>, class_launch(tab.grid), dim3(256), 0,
+                               stream, x, d, rhs, ax, inv_diag, tab.table, tab.ok, T(c1),
+                               T(c2), k == 0 ? 1 : 0, 0, tab.grid);
+          else
+            hipLaunchKernelGGL(chebyshev_step_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0,
+                               stream, x, d, rhs, ax, inv_diag, T(c1), T(c2), k == 0 ? 1 : 0,
+                               n);
         }
     }
 
@@ -2443,6 +2555,25 @@ namespace ppf
         return l.vf[i].get();
       else
         return l.vd[i].get();
+    }
+
+    /// The class table of level l's inverse diagonal (empty until setup_level built it).
+    template <typename T>
+    DiagTable<T>
+    level_table(Level &l, int n_fields)
+    {
+      DiagTable<T> t;
+      const T     *table = nullptr;
+      if constexpr (std::is_same_v<T, float>)
+        table = l.table_f.get();
+      else
+        table = l.table_d.get();
+      if (!table || !l.table_ok.get())
+        return t;
+      t.table = table;
+      t.ok    = l.table_ok.get();
+      t.grid  = class_grid(l.grid, n_fields);
+      return t;
     }
 
     template <typename T>
@@ -2561,13 +2692,9 @@ namespace ppf
     void
     transfer_fields_to_levels(int b, LinearState &ls)
     {
+      // The level copies keep their buffers from one solve to the next (the block's
+      // dependencies do not change), so no solve allocates or frees device memory.
       const auto &deps = blocks[b].dependencies_lhs;
-      for (auto &l : ls.levels)
-        for (int f = 0; f < PPF_MAX_FIELDS; ++f)
-          {
-            l.field_d[f].clear();
-            l.field_f[f].clear();
-          }
       for (const auto &in : declared_inputs(deps))
         {
           if (in.slot == prismspf::dependency_slot(prismspf::SRC))
@@ -2603,6 +2730,12 @@ namespace ppf
       const auto   &lp = lin_params(b);
       const int64_t n  = int64_t(blocks[b].field_indices.size()) * l.grid.n_nodes;
       inverse_diagonal<T>(b, level_fields<T>(l), level_scaling(b), linv<T>(l));
+      if constexpr (std::is_same_v<T, float>)
+        build_diag_table<float>(linv<T>(l), l.grid, int(blocks[b].field_indices.size()),
+                                l.table_f, l.table_ok);
+      else
+        build_diag_table<double>(linv<T>(l), l.grid, int(blocks[b].field_indices.size()),
+                                 l.table_d, l.table_ok);
       if (l.eigenvalues_estimated)
         return;
       estimate_eigenvalues<T>([&](T *dst, const T *src) { level_vmult<T>(b, l, dst, src); },
@@ -2618,17 +2751,116 @@ namespace ppf
       LinearState &ls  = linear_state(b);
       Level       &top = ls.levels.back();
       if (top.precision)
-        {
-          convert(lvec<float>(top, 1), g, ls.n);
-          v_cycle<float>(b, int(ls.levels.size()) - 1);
-          convert(h, lvec<float>(top, 0), ls.n);
-        }
+        convert(lvec<float>(top, 1), g, ls.n);
       else
+        copy(lvec<double>(top, 1), g, ls.n);
+      run_vcycle(b);
+      if (top.precision)
+        convert(h, lvec<float>(top, 0), ls.n);
+      else
+        copy(h, lvec<double>(top, 0), ls.n);
+    }
+
+    /// One V-cycle on the top level's defect vector.  With `mg_graphs` set, the launch
+    /// sequence is captured into a HIP graph once an eager V-cycle has shown that it
+    /// needs no host synchronization and allocates nothing, and then replayed: the
+    /// V-cycle's ~150 launches (7 levels) go to the GPU as one submission.  The graph is
+    /// captured again after anything that changes the launches' arguments: a new split
+    /// plan or option (capture epoch), a reallocated buffer (allocation count), another
+    /// halt flag (increment parity) or another level precision.  V-cycles whose coarse
+    /// solve is driven from the host stay eager.
+    void
+    run_vcycle(int b)
+    {
+      LinearState &ls    = linear_state(b);
+      Level       &top   = ls.levels.back();
+      const int    li    = int(ls.levels.size()) - 1;
+      auto         eager = [&] {
+        if (top.precision)
+          v_cycle<float>(b, li);
+        else
+          v_cycle<double>(b, li);
+      };
+      if (!mg_graphs || ls.vg_off || (probe_req.pending && probe_req.block == b))
         {
-          copy(lvec<double>(top, 1), g, ls.n);
-          v_cycle<double>(b, int(ls.levels.size()) - 1);
-          copy(h, lvec<double>(top, 0), ls.n);
+          eager();
+          return;
         }
+      const long allocs = device_allocations();
+      for (int s = 0; s < 2; ++s)
+        if (ls.vg_exec[s] && ls.vg_halt[s] == enqueue_halt && ls.vg_epoch[s] == graph_epoch &&
+            ls.vg_alloc[s] == allocs && ls.vg_prec[s] == top.precision)
+          {
+            PPF_HIP_CHECK(hipGraphLaunch(ls.vg_exec[s], stream));
+            return;
+          }
+      if (ls.vg_warm != graph_epoch)
+        {
+          const long syncs = sync_count_;
+          eager();
+          if (sync_count_ != syncs)
+            ls.vg_off = true;
+          else if (device_allocations() == allocs)
+            ls.vg_warm = graph_epoch;
+          if (std::getenv("PPF_VERBOSE"))
+            std::fprintf(stderr, "block %d: eager V-cycle (epoch %ld, syncs %ld, allocations "
+                                 "%ld -> %ld)\n",
+                         b, graph_epoch, sync_count_ - syncs, allocs, long(device_allocations()));
+          return;
+        }
+      const int  slot  = ls.vg_next;
+      hipGraph_t graph = nullptr;
+      if (std::getenv("PPF_VERBOSE"))
+        std::fprintf(stderr, "block %d: capturing the V-cycle (epoch %ld, allocations %ld)\n",
+                     b, graph_epoch, allocs);
+      PPF_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+      bool captured = true;
+      try
+        {
+          eager();
+        }
+      catch (...)
+        {
+          captured = false;
+        }
+      const hipError_t end = hipStreamEndCapture(stream, &graph);
+      hipGraphExec_t   exec = nullptr;
+      if (captured && end == hipSuccess && graph && ls.vg_exec[slot])
+        {
+          // same launch sequence with new arguments: update the instantiated graph
+          hipGraphNode_t              bad = nullptr;
+          hipGraphExecUpdateResult    res {};
+          if (hipGraphExecUpdate(ls.vg_exec[slot], graph, &bad, &res) == hipSuccess &&
+              res == hipGraphExecUpdateSuccess)
+            exec = ls.vg_exec[slot];
+          else
+            {
+              (void) hipGetLastError();
+              (void) hipGraphExecDestroy(ls.vg_exec[slot]);
+              ls.vg_exec[slot] = nullptr;
+            }
+        }
+      if (captured && end == hipSuccess && graph &&
+          (exec || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess))
+        {
+          ls.vg_exec[slot]  = exec;
+          ls.vg_halt[slot]  = enqueue_halt;
+          ls.vg_epoch[slot] = graph_epoch;
+          ls.vg_alloc[slot] = allocs;
+          ls.vg_prec[slot]  = top.precision;
+          ls.vg_next        = slot ^ 1;
+        }
+      if (graph)
+        (void) hipGraphDestroy(graph);
+      if (!exec)
+        {
+          // not capturable after all: this block's V-cycles stay eager
+          (void) hipGetLastError();
+          ls.vg_off = true;
+          eager();
+          return;
+        }
+      PPF_HIP_CHECK(hipGraphLaunch(exec, stream));
     }
 
     /// Multigrid::level_v_step.  Level vectors: 0 solution, 1 defect, 2 scratch t,
@@ -2648,8 +2880,9 @@ namespace ppf
           return;
         }
       const unsigned int deg = lp.chebyshev_parameters.degree;
+      const DiagTable<T> tab = level_table<T>(l, int(blocks[b].field_indices.size()));
       chebyshev<T>(A, linv<T>(l), n, deg, l.lambda_max, l.lambda_min, lvec<T>(l, 0),
-                   lvec<T>(l, 1), lvec<T>(l, 3), lvec<T>(l, 4), true);
+                   lvec<T>(l, 1), lvec<T>(l, 3), lvec<T>(l, 4), true, tab);
       // t = defect - A u
       A(lvec<T>(l, 2), lvec<T>(l, 0));
       axpby(lvec<T>(l, 2), T(1), lvec<T>(l, 1), T(-1), n);
@@ -2669,7 +2902,7 @@ namespace ppf
                                       dirichlet_mask(f));
         }
       chebyshev<T>(A, linv<T>(l), n, deg, l.lambda_max, l.lambda_min, lvec<T>(l, 0),
-                   lvec<T>(l, 1), lvec<T>(l, 3), lvec<T>(l, 4), false);
+                   lvec<T>(l, 1), lvec<T>(l, 3), lvec<T>(l, 4), false, tab);
     }
 
     /// Scratch lattices of the separable transfers between level l and the next
@@ -2703,20 +2936,30 @@ namespace ppf
         {
           // z: (c0, c1, c2) -> (c0, c1, f2)
           T *t1 = transfer_tmp<T>(l, 1, int64_t(c0) * c1 * f2);
-          hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, false, 0>),
-                             dim3(grid_for(int64_t(c0) * c1 * f2, 256)), dim3(256), 0, stream,
-                             t1, in, c0, c1, f2, c0, c1, 2, gc, gf, 0u);
+          hipLaunchKernelGGL((transfer_axis_kernel<dim, degree, T, false, 0, 2>),
+                             transfer_grid(c0, c1, f2), dim3(256), 0, stream, t1, in, c0, c1,
+                             c0, c1, gc, gf, 0u);
           in = t1;
         }
       // y: (c0, c1, f2) -> (c0, f1, f2)
       T *t0 = transfer_tmp<T>(l, 0, int64_t(c0) * f1 * f2);
-      hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, false, 0>),
-                         dim3(grid_for(int64_t(c0) * f1 * f2, 256)), dim3(256), 0, stream, t0,
-                         in, c0, f1, f2, c0, c1, 1, gc, gf, 0u);
+      hipLaunchKernelGGL((transfer_axis_kernel<dim, degree, T, false, 0, 1>),
+                         transfer_grid(c0, f1, f2), dim3(256), 0, stream, t0, in, c0, f1, c0,
+                         c1, gc, gf, 0u);
       // x: (c0, f1, f2) -> fine, added
-      hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, false, 1>),
-                         dim3(grid_for(gf.n_nodes, 256)), dim3(256), 0, stream, fine_v, t0, f0,
-                         f1, f2, c0, f1, 0, gc, gf, mask);
+      hipLaunchKernelGGL((transfer_axis_kernel<dim, degree, T, false, 1, 0>),
+                         transfer_grid(f0, f1, f2), dim3(256), 0, stream, fine_v, t0, f0, f1,
+                         c0, f1, gc, gf, mask);
+    }
+
+    /// Launch grid of transfer_axis_kernel for an e0 x e1 x e2 output lattice.
+    static dim3
+    transfer_grid(int e0, int e1, int e2)
+    {
+      const int64_t plane = int64_t(e0) * e1;
+      if (plane >= (int64_t(1) << 31) || e2 > 65535)
+        throw Error(status_unsupported, "transfer: lattice too large for one launch");
+      return dim3(grid_for(plane, 256), unsigned(e2));
     }
 
     /// coarse = P^T fine (coarse Dirichlet nodes zero), one pass per axis.
@@ -2730,26 +2973,26 @@ namespace ppf
       const int f0 = gf.nodes[0], f1 = gf.nodes[1], f2 = gf.nodes[2];
       // x: (f0, f1, f2) -> (c0, f1, f2)
       T *t0 = transfer_tmp<T>(l, 0, int64_t(c0) * f1 * f2);
-      hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, true, 0>),
-                         dim3(grid_for(int64_t(c0) * f1 * f2, 256)), dim3(256), 0, stream, t0,
-                         fine_v, c0, f1, f2, f0, f1, 0, gf, gc, 0u);
+      hipLaunchKernelGGL((transfer_axis_kernel<dim, degree, T, true, 0, 0>),
+                         transfer_grid(c0, f1, f2), dim3(256), 0, stream, t0, fine_v, c0, f1,
+                         f0, f1, gf, gc, 0u);
       if constexpr (dim == 3)
         {
           // y: (c0, f1, f2) -> (c0, c1, f2)
           T *t1 = transfer_tmp<T>(l, 1, int64_t(c0) * c1 * f2);
-          hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, true, 0>),
-                             dim3(grid_for(int64_t(c0) * c1 * f2, 256)), dim3(256), 0, stream,
-                             t1, t0, c0, c1, f2, c0, f1, 1, gf, gc, 0u);
+          hipLaunchKernelGGL((transfer_axis_kernel<dim, degree, T, true, 0, 1>),
+                             transfer_grid(c0, c1, f2), dim3(256), 0, stream, t1, t0, c0, c1,
+                             c0, f1, gf, gc, 0u);
           // z: (c0, c1, f2) -> coarse, Dirichlet nodes zero
-          hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, true, 2>),
-                             dim3(grid_for(gc.n_nodes, 256)), dim3(256), 0, stream, coarse_v, t1,
-                             c0, c1, c2, c0, c1, 2, gf, gc, mask);
+          hipLaunchKernelGGL((transfer_axis_kernel<dim, degree, T, true, 2, 2>),
+                             transfer_grid(c0, c1, c2), dim3(256), 0, stream, coarse_v, t1,
+                             c0, c1, c0, c1, gf, gc, mask);
         }
       else
         // y: (c0, f1) -> coarse, Dirichlet nodes zero
-        hipLaunchKernelGGL((transfer_pass_kernel<dim, degree, T, true, 2>),
-                           dim3(grid_for(gc.n_nodes, 256)), dim3(256), 0, stream, coarse_v, t0,
-                           c0, c1, c2, c0, f1, 1, gf, gc, mask);
+        hipLaunchKernelGGL((transfer_axis_kernel<dim, degree, T, true, 2, 1>),
+                             transfer_grid(c0, c1, c2), dim3(256), 0, stream, coarse_v, t0,
+                             c0, c1, c0, f1, gf, gc, mask);
     }
 
     template <typename T>
@@ -3207,6 +3450,8 @@ namespace ppf
     /// launches whose arrays exceed stencil_nt_min_bytes).
     StorePolicy                            stencil_store_policy = StorePolicy::Auto;
     int64_t                                stencil_nt_min_bytes = int64_t(192) << 20;
+    /// Lattices with fewer nodes run split operators node by node (launch_stencil).
+    int64_t                                stencil_min_nodes    = 100000;
     /// Start every linear solve from zero instead of the previous solution (which the
     /// reference keeps as the next initial guess, linear_solver.h:330-336).
     bool                                   zero_initial_guess    = false;
@@ -3260,6 +3505,11 @@ namespace ppf
     } probe_req;
     DeviceBuffer<int>                              halt_flag;
     const int                                     *enqueue_halt = nullptr;
+    /// Capture V-cycles into HIP graphs (run_vcycle); graph_epoch changes whenever a
+    /// captured launch's arguments may have changed.
+    bool                                           mg_graphs   = true;
+    long                                           graph_epoch = 0;
+    long                                           sync_count_ = 0;
     std::string                                    log;
     std::array<bool, PPF_MAX_FIELDS>               rotated {};
   };

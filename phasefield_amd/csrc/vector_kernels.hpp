@@ -464,6 +464,135 @@ namespace ppf
     x[i] += di;
   }
 
+  // Inverse diagonal by node class.  For constant-coefficient split operators (the
+  // Poisson and level operators) the diagonal is a function of each axis' node class
+  // only: 0 odd (Q2), 1 even interior (or periodic), 2 lower and 3 upper non-periodic
+  // boundary.  The smoother then reads a 4^dim table per field instead of a diagonal
+  // array (one stream fewer per step).  The table is checked node by node against the
+  // computed diagonal whenever it is rebuilt; where any node differs the kernels read
+  // the diagonal array as before (the flag is read once per thread, no host sync).
+
+  /// Lattice of a diagonal table: nodes and periodicity per axis, fields, degree.
+  struct ClassGrid
+  {
+    int     n[3];
+    int     periodic[3];
+    int     degree;
+    int     dim;
+    int     n_fields;
+    int64_t n_nodes;
+  };
+
+  constexpr int diag_classes = 64;
+
+  PPF_HD PPF_INLINE int
+  axis_class(const ClassGrid &g, int d, int i)
+  {
+    if (d >= g.dim)
+      return 1;
+    if (!g.periodic[d])
+      {
+        if (i == 0)
+          return 2;
+        if (i == g.n[d] - 1)
+          return 3;
+      }
+    return (g.degree == 2 && (i & 1)) ? 0 : 1;
+  }
+
+  /// Representative coordinate of class c along axis d, -1 if the class does not occur.
+  PPF_HD PPF_INLINE int
+  class_representative(const ClassGrid &g, int d, int c)
+  {
+    if (d >= g.dim)
+      return c == 1 ? 0 : -1;
+    const int n = g.n[d];
+    int       i = -1;
+    if (c == 0)
+      i = (g.degree == 2 && n >= 2) ? 1 : -1;
+    else if (c == 1)
+      i = g.periodic[d] ? 0 : (g.degree == 2 ? 2 : 1);
+    else if (c == 2)
+      i = g.periodic[d] ? -1 : 0;
+    else
+      i = (g.periodic[d] || n < 2) ? -1 : n - 1;
+    return (i >= 0 && i < n && axis_class(g, d, i) == c) ? i : -1;
+  }
+
+  /// table[f * 64 + class] from the diagonal at one node of each occurring class; one
+  /// workgroup of n_fields * 64 threads; sets *ok = 1.
+  template <typename T>
+  __global__ void
+  diag_table_build_kernel(T *__restrict__ table, int *__restrict__ ok,
+                          const T *__restrict__ inv_diag, const ClassGrid g)
+  {
+    const int t = threadIdx.x;
+    if (t == 0)
+      *ok = 1;
+    if (t >= g.n_fields * diag_classes)
+      return;
+    const int f = t / diag_classes, c = t % diag_classes;
+    const int i0 = class_representative(g, 0, c & 3);
+    const int i1 = class_representative(g, 1, (c >> 2) & 3);
+    const int i2 = class_representative(g, 2, c >> 4);
+    T         v  = T(0);
+    if (i0 >= 0 && i1 >= 0 && i2 >= 0)
+      v = inv_diag[f * g.n_nodes + i0 + int64_t(g.n[0]) * (i1 + int64_t(g.n[1]) * i2)];
+    table[t] = v;
+  }
+
+  PPF_HD PPF_INLINE int
+  node_class(const ClassGrid &g, int i0, int i1, int i2)
+  {
+    return axis_class(g, 0, i0) + 4 * axis_class(g, 1, i1) + 16 * axis_class(g, 2, i2);
+  }
+
+  /// Clears *ok where a node's diagonal differs from its class entry.  Grid: x over the
+  /// n0 * n1 plane, y over n2, z over the fields.
+  template <typename T>
+  __global__ void
+  diag_table_check_kernel(int *__restrict__ ok, const T *__restrict__ table,
+                          const T *__restrict__ inv_diag, const ClassGrid g)
+  {
+    const unsigned int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= unsigned(g.n[0]) * unsigned(g.n[1]))
+      return;
+    const int     i0 = int(r % unsigned(g.n[0])), i1 = int(r / unsigned(g.n[0]));
+    const int     i2 = int(blockIdx.y), f = int(blockIdx.z);
+    const int64_t i  = f * g.n_nodes + int64_t(i2) * g.n[0] * g.n[1] + r;
+    if (inv_diag[i] != table[f * diag_classes + node_class(g, i0, i1, i2)])
+      *ok = 0;
+  }
+
+  /// chebyshev_step_kernel / chebyshev_first_zero_kernel (first_zero set) with the
+  /// inverse diagonal taken from the class table when *ok (grid as the check kernel).
+  template <typename T>
+  __global__ void
+  chebyshev_table_kernel(T *__restrict__ x, T *__restrict__ d, const T *__restrict__ b,
+                         const T *__restrict__ ax, const T *__restrict__ inv_diag,
+                         const T *__restrict__ table, const int *__restrict__ ok, T c1, T c2,
+                         int first, int first_zero, const ClassGrid g)
+  {
+    const unsigned int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= unsigned(g.n[0]) * unsigned(g.n[1]))
+      return;
+    const int     i0 = int(r % unsigned(g.n[0])), i1 = int(r / unsigned(g.n[0]));
+    const int     i2 = int(blockIdx.y), f = int(blockIdx.z);
+    const int64_t i  = f * g.n_nodes + int64_t(i2) * g.n[0] * g.n[1] + r;
+    const T inv = *ok ? table[f * diag_classes + node_class(g, i0, i1, i2)] : inv_diag[i];
+    if (first_zero)
+      {
+        const T di = c2 * inv * b[i];
+        d[i]       = di;
+        x[i]       = di;
+        return;
+      }
+    const T rr = b[i] - ax[i];
+    const T di = first ? c2 * inv * rr : c1 * d[i] + c2 * inv * rr;
+    d[i]       = di;
+    x[i] += di;
+  }
+
   /// NaN/Inf probe of entry 0 (reference problem.cc:453-469 checks one entry per field).
   template <typename T>
   __global__ void
@@ -591,120 +720,35 @@ namespace ppf
   //   p = 2: even I -> coarse I/2 (1); odd I in coarse cell c = I/4 at xi = 1/4 or 3/4
   //          -> local nodes (0,1,2) with (3/8, 3/4, -1/8) or (-1/8, 3/4, 3/8).
 
-  template <int degree>
-  PPF_HD PPF_INLINE int
-  prolongation_stencil_1d(const GridDesc &coarse, int d, int I, int (&J)[3], double (&w)[3])
-  {
-    if ((I & 1) == 0)
-      {
-        J[0] = I / 2;
-        w[0] = 1.0;
-        return 1;
-      }
-    if constexpr (degree == 1)
-      {
-        J[0] = (I - 1) / 2;
-        J[1] = wrap_lattice(coarse, d, (I + 1) / 2);
-        w[0] = 0.5;
-        w[1] = 0.5;
-        return 2;
-      }
-    else
-      {
-        const int c    = I / 4;
-        const int base = 2 * c;
-        J[0]           = base;
-        J[1]           = base + 1;
-        J[2]           = wrap_lattice(coarse, d, base + 2);
-        if ((I & 3) == 1)
-          {
-            w[0] = 3.0 / 8.0;
-            w[1] = 3.0 / 4.0;
-            w[2] = -1.0 / 8.0;
-          }
-        else
-          {
-            w[0] = -1.0 / 8.0;
-            w[1] = 3.0 / 4.0;
-            w[2] = 3.0 / 8.0;
-          }
-        return 3;
-      }
-  }
-
-  /// Transpose of prolongation_stencil_1d: fine lattice coordinates feeding coarse J.
-  template <int degree>
-  PPF_HD PPF_INLINE int
-  restriction_stencil_1d(const GridDesc &fine, int d, int J, int (&I)[5], double (&w)[5])
-  {
-    int    cand[5];
-    double cw[5];
-    int    m = 0;
-    if constexpr (degree == 1)
-      {
-        cand[m] = 2 * J, cw[m++] = 1.0;
-        cand[m] = 2 * J - 1, cw[m++] = 0.5;
-        cand[m] = 2 * J + 1, cw[m++] = 0.5;
-      }
-    else
-      {
-        if ((J & 1) == 0)
-          {
-            cand[m] = 2 * J, cw[m++] = 1.0;
-            cand[m] = 2 * J - 1, cw[m++] = 3.0 / 8.0;
-            cand[m] = 2 * J + 1, cw[m++] = 3.0 / 8.0;
-            cand[m] = 2 * J - 3, cw[m++] = -1.0 / 8.0;
-            cand[m] = 2 * J + 3, cw[m++] = -1.0 / 8.0;
-          }
-        else
-          {
-            cand[m] = 2 * J, cw[m++] = 1.0;
-            cand[m] = 2 * J - 1, cw[m++] = 3.0 / 4.0;
-            cand[m] = 2 * J + 1, cw[m++] = 3.0 / 4.0;
-          }
-      }
-    int n = 0;
-    for (int a = 0; a < m; ++a)
-      {
-        int i = cand[a];
-        if (fine.periodic[d])
-          {
-            const int period = fine.nodes[d];
-            i                = (i % period + period) % period;
-          }
-        else if (i < 0 || i >= fine.nodes[d])
-          continue;
-        I[n] = i;
-        w[n] = cw[a];
-        ++n;
-      }
-    return n;
-  }
-
   /// One axis of a separable transfer between nested lattices (prolongation and
   /// restriction are tensor products of the 1-D operators above, so they run as one
   /// pass per axis; far fewer loads than the 3^d / 5^d gathers).  `in` and `out` are
-  /// lattices of extents ei / eo (x fastest), equal except along `axis`.  Prolongation
-  /// passes (RESTRICT false) index the output by fine coordinates and take their
-  /// weights from prolongation_stencil_1d on the coarse grid `g1d`; restriction passes
-  /// index the output by coarse coordinates with restriction_stencil_1d on the fine
-  /// grid.  The last pass of a prolongation adds into the fine vector except at its
-  /// Dirichlet nodes (MODE 1); the last pass of a restriction zeroes the coarse
-  /// Dirichlet nodes (MODE 2); intermediate passes store (MODE 0).  `gout` is the
-  /// output grid for MODE 1 and 2 (storage index and Dirichlet test).
-  template <int dim, int degree, typename T, bool RESTRICT, int MODE>
+  /// lattices of extents ei / eo (x fastest), equal except along AXIS.  Prolongation
+  /// passes (RESTRICT false) index the output by fine coordinates I and take the 1-D
+  /// weights above on the coarse grid `g1d`; restriction passes index the output by
+  /// coarse coordinates J and gather the transposed weights from fine positions 2J,
+  /// 2J -+ 1 (and 2J -+ 3 for p = 2, even J) of the fine grid `g1d`, dropping positions
+  /// outside a non-periodic lattice.  Every output takes a fixed number of taps, an
+  /// absent one adding an exact zero.  The last pass of a prolongation adds into the
+  /// fine vector except at its Dirichlet nodes (MODE 1); the last pass of a restriction
+  /// zeroes the coarse Dirichlet nodes (MODE 2); intermediate passes store (MODE 0).
+  /// `gout` is the output grid for MODE 1 and 2 (storage index and Dirichlet test).
+  /// Grid: x over the output plane (eo0 * eo1 < 2^31), y over its planes.
+  template <int dim, int degree, typename T, bool RESTRICT, int MODE, int AXIS>
   __global__ void
-  transfer_pass_kernel(T *__restrict__ out, const T *__restrict__ in, int eo0, int eo1,
-                       int eo2, int ei0, int ei1, int axis, const GridDesc g1d,
-                       const GridDesc gout, unsigned int mask)
+  transfer_axis_kernel(T *__restrict__ out, const T *__restrict__ in, int eo0, int eo1,
+                       int ei0, int ei1, const GridDesc g1d, const GridDesc gout,
+                       unsigned int mask)
   {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= int64_t(eo0) * eo1 * eo2)
+    const unsigned int r     = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned int plane = unsigned(eo0) * unsigned(eo1);
+    if (r >= plane)
       return;
     int o[3];
-    o[0] = int(t % eo0);
-    o[1] = int((t / eo0) % eo1);
-    o[2] = int(t / (int64_t(eo0) * eo1));
+    o[0]            = int(r % unsigned(eo0));
+    o[1]            = int(r / unsigned(eo0));
+    o[2]            = int(blockIdx.y);
+    const int64_t t = int64_t(o[2]) * plane + r;
     if constexpr (MODE == 1)
       if (is_dirichlet<dim>(gout, o, mask))
         return;
@@ -714,34 +758,81 @@ namespace ppf
           out[t] = T(0);
           return;
         }
-    int    idx[5];
-    double w[5];
-    int    m;
+    // input offset of the output's position with the AXIS coordinate zero
+    int c[3] = {o[0], o[1], o[2]};
+    c[AXIS]  = 0;
+    const int64_t base   = c[0] + int64_t(ei0) * (c[1] + int64_t(ei1) * c[2]);
+    const int64_t stride = AXIS == 0 ? 1 : (AXIS == 1 ? int64_t(ei0) : int64_t(ei0) * ei1);
+    const int     n      = g1d.nodes[AXIS];
+    const bool    per    = g1d.periodic[AXIS] != 0;
+    // Tap positions stay within one period of the lattice (a finer periodic lattice has
+    // at least 2 cells), so one wrap suffices.
+    const auto    tap    = [&](int i, bool &ok) {
+      if (per)
+        i = i < 0 ? i + n : (i >= n ? i - n : i);
+      else
+        ok = ok && i >= 0 && i < n;
+      return ok ? i : 0;
+    };
+    const int a = o[AXIS];
+    T         sum = T(0);
     if constexpr (RESTRICT)
-      m = restriction_stencil_1d<degree>(g1d, axis, o[axis], idx, w);
-    else
       {
-        int    j3[3];
-        double w3[3];
-        m = prolongation_stencil_1d<degree>(g1d, axis, o[axis], j3, w3);
-        for (int a = 0; a < m; ++a)
+        // taps in the order 2J, 2J - 1, 2J + 1, 2J - 3, 2J + 3
+        constexpr int K = degree == 1 ? 3 : 5;
+        int           off[K];
+        T             w[K];
+        if constexpr (degree == 1)
           {
-            idx[a] = j3[a];
-            w[a]   = w3[a];
+            off[0] = 0, off[1] = -1, off[2] = 1;
+            w[0] = T(1.0), w[1] = T(0.5), w[2] = T(0.5);
+          }
+        else
+          {
+            const bool even = (a & 1) == 0;
+            off[0] = 0, off[1] = -1, off[2] = 1, off[3] = -3, off[4] = 3;
+            w[0] = T(1.0);
+            w[1] = w[2] = even ? T(3.0 / 8.0) : T(3.0 / 4.0);
+            w[3] = w[4] = even ? T(-1.0 / 8.0) : T(0);
+          }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          {
+            bool      ok = degree == 1 || k < 3 || (a & 1) == 0;
+            const int i  = tap(2 * a + off[k], ok);
+            const T   x  = in[base + i * stride];
+            sum += ok ? w[k] * x : T(0);
           }
       }
-    int c[3] = {o[0], o[1], o[2]};
-    T   sum  = T(0);
-    for (int a = 0; a < m; ++a)
+    else
       {
-        c[axis] = idx[a];
-        sum += T(w[a]) * in[c[0] + int64_t(ei0) * (c[1] + int64_t(ei1) * c[2])];
+        // taps in the order of the coarse local nodes
+        if ((a & 1) == 0)
+          sum += T(1.0) * in[base + (a / 2) * stride];
+        else if constexpr (degree == 1)
+          {
+            bool      ok = true;
+            const int j1 = tap((a + 1) / 2, ok);
+            sum += T(0.5) * in[base + ((a - 1) / 2) * stride];
+            sum += T(0.5) * in[base + j1 * stride];
+          }
+        else
+          {
+            const int  b0 = 2 * (a / 4);
+            bool       ok = true;
+            const int  j2 = tap(b0 + 2, ok);
+            const bool q1 = (a & 3) == 1;
+            sum += (q1 ? T(3.0 / 8.0) : T(-1.0 / 8.0)) * in[base + b0 * stride];
+            sum += T(3.0 / 4.0) * in[base + (b0 + 1) * stride];
+            sum += (q1 ? T(-1.0 / 8.0) : T(3.0 / 8.0)) * in[base + j2 * stride];
+          }
       }
     if constexpr (MODE == 1)
       out[t] += sum;
     else
       out[t] = sum;
   }
+
 
   /// Nodal injection fine -> coarse (coarse lattice J = fine lattice stride*J, stride a
   /// power of two); transfers non-operand fields to the levels

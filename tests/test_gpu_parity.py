@@ -61,9 +61,12 @@ OPERATOR_CASES = [
 
 
 @pytest.mark.parametrize("app,text,ops", OPERATOR_CASES, ids=[c[0] for c in OPERATOR_CASES])
-@pytest.mark.parametrize("path", ["auto", "generic", "generic_two_pass", "split_node"])
+@pytest.mark.parametrize("path", ["auto", "auto_node", "generic", "generic_two_pass",
+                                  "split_node"])
 def test_operator_matches_oracle(app, text, ops, path):
-    """auto: tiled stencil kernel for split hooks, generic cell path otherwise;
+    """auto: tiled stencil kernel for split hooks (at any lattice size), generic cell
+    path otherwise; auto_node: the default routing, in which split hooks on lattices
+    below 100000 nodes run node by node;
     generic: cell path for every hook (one-pass cell-block kernel); generic_two_pass: the
     cell path's two-pass kernels (per-cell results through a global array, then a node
     gather); split_node: one-thread-per-node split kernel (hooks without the split
@@ -77,6 +80,11 @@ def test_operator_matches_oracle(app, text, ops, path):
     if path == "generic_two_pass":
         ctx.set_option("generic_kernel", "two_pass")
         path = "generic"
+    if path == "auto_node":
+        # split hooks on these small lattices run node by node by default
+        path = "auto"
+    else:
+        ctx.set_option("stencil_min_nodes", 0)  # auto: the tiled kernel at any size
     ctx.set_option("operator_path", path)
     for block_id, which, nb in ops:
         for scaling in (SCALE_NONE, SCALE_INVM, SCALE_INVM_SQRT):
@@ -194,6 +202,7 @@ def test_stencil_tilings_match_oracle(app, text, ops):
     info = ctx.info
     rng = np.random.default_rng(21)
     seed_fields(ctx, orc, rng)
+    ctx.set_option("stencil_min_nodes", 0)
     for i, (waves, rows, zc) in enumerate(TILINGS):
         ctx.set_option("stencil_waves", waves)
         ctx.set_option("stencil_rows", rows)

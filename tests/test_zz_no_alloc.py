@@ -72,6 +72,7 @@ def test_noalloc_matches_operator_two_calls(device, app, texts, block, which, sc
         ctx = Context(app, text)
         ctx.set_option("operator_path", path)
         ctx.set_option("generic_kernel", kernel)
+        ctx.set_option("stencil_min_nodes", 0)  # split hooks: the tiled kernel
         seeded(ctx, rng)
         print(app, ctx.info.n_nodes, flush=True)
         a, b = both(ctx, block, which, scaling, prec, nb, rng)
