@@ -120,8 +120,11 @@ int ppf_field_name(ppf_ctx *ctx, int field, char *buf, int len);
  * "linear_tolerance" = <value> and "linear_max_iterations" = <n> (applied to every
  * linear block, overriding the parameter file); "linear_initial_guess" =
  * previous|zero (previous: the last solution, as the reference); "explicit_precision" =
- * double|single (single: explicit blocks compute in FP32 from FP32 copies of their
- * inputs and store into the FP64 state). */
+ * double|single|single_copies (single: explicit blocks compute in FP32 on the FP64
+ * state: the stencil kernel rounds its FP64 loads to FP32 in registers and stores FP64,
+ * after a first increment through FP32 copies that checks the split operator; blocks
+ * the stencil kernel does not take, and single_copies always, compute from FP32 copies
+ * of their inputs and convert the results into the FP64 state). */
 int ppf_set_option(ppf_ctx *ctx, const char *key, const char *value);
 
 /* Field state `age` (0 = current, k = old_k, PPF_AGE_LATEST = the newest values) from/to

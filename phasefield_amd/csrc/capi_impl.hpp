@@ -420,9 +420,11 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
       }
     else if (k == "explicit_precision")
       {
-        if (v != "double" && v != "single")
-          throw ppf::Error(ppf::status_invalid, "explicit_precision must be double|single");
-        c.rt.explicit_fp32 = v == "single";
+        if (v != "double" && v != "single" && v != "single_copies")
+          throw ppf::Error(ppf::status_invalid,
+                           "explicit_precision must be double|single|single_copies");
+        c.rt.explicit_fp32        = v != "double";
+        c.rt.explicit_fp32_copies = v == "single_copies";
       }
     else if (k == "write_output")
       {

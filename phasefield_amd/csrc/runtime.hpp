@@ -537,12 +537,13 @@ namespace ppf
       PPF_HIP_CHECK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToDevice, stream));
     }
 
+    /// dst = src converted on the stream; skipped on the device once *halt is set.
     template <typename T, typename U>
     void
-    convert(T *dst, const U *src, int64_t n)
+    convert(T *dst, const U *src, int64_t n, const int *halt = nullptr)
     {
       hipLaunchKernelGGL((convert_kernel<T, U>), dim3(grid_for(n, 256)), dim3(256), 0, stream,
-                         dst, src, n);
+                         dst, src, n, halt);
     }
 
     template <typename T>
@@ -1072,10 +1073,12 @@ namespace ppf
 
     /// Launches the tiled stencil kernel for a split operator; false if the operator
     /// has more stencil operands than the kernel stages (the per-node kernel runs it).
-    template <typename number, int id>
+    /// The kernel computes in `number` and loads and stores arrays of `ST` (ST = double
+    /// with number = float: FP32 arithmetic on the FP64 state, no converted copies).
+    template <typename number, int id, typename ST = number>
     bool
-    launch_stencil(int b, bool lhs, const GridDesc &g, const DepTable<number> &deps,
-                   const OutTable<number> &out, const SplitPlan &plan)
+    launch_stencil(int b, bool lhs, const GridDesc &g, const DepTable<ST> &deps,
+                   const OutTable<ST> &out, const SplitPlan &plan)
     {
       // Small lattices (multigrid's coarse levels) go to the one-thread-per-node kernel:
       // a tile's march over its z planes would take longer than the whole lattice node
@@ -1083,7 +1086,7 @@ namespace ppf
       if (g.n_nodes < stencil_min_nodes && !coarse_pending &&
           !(probe_req.pending && probe_req.block == b))
         return false;
-      StencilArgs<number> a {};
+      StencilArgs<number, ST> a {};
       // Distinct (field, slot) operands of the gradient terms.
       int op_field[stencil_max_ops], op_slot[stencil_max_ops];
       a.n_ops = 0;
@@ -1112,7 +1115,7 @@ namespace ppf
         }
       // Unused operand and value entries point at an array of the lattice size so the
       // kernel loads them without a branch (their values are never used).
-      const number *any_array = nullptr;
+      const ST *any_array = nullptr;
       for (int f = 0; f < PPF_MAX_FIELDS && !any_array; ++f)
         if (out.out_mask & (1u << f))
           any_array = out.dst[f];
@@ -1122,7 +1125,7 @@ namespace ppf
       // zeros so that its (unused) sums stay finite; operands beyond the instantiated
       // count are never read and point at any lattice-size array.
       for (int k = a.n_ops; k < stencil_max_ops; ++k)
-        a.op[k] = k == 0 ? zero_array<number>(g.n_nodes) : any_array;
+        a.op[k] = k == 0 ? zero_array<ST>(g.n_nodes) : any_array;
       for (int f = 0; f < PPF_MAX_FIELDS; ++f)
         for (int s = 0; s < PPF_N_SLOTS; ++s)
           {
@@ -1179,7 +1182,7 @@ namespace ppf
             a.any_dmask |= out.dirichlet_mask[f];
         }
       // 32-bit byte offsets in the kernel: arrays up to 4 GiB.
-      if (int64_t(a.n[0]) * a.n[1] * a.n[2] * int64_t(sizeof(number)) > (int64_t(1) << 32))
+      if (int64_t(a.n[0]) * a.n[1] * a.n[2] * int64_t(sizeof(ST)) > (int64_t(1) << 32))
         return false;
       // Store policy: non-temporal when the distinct arrays the launch reads and writes
       // exceed 192 MiB, i.e. cannot stay in the 256 MB Infinity Cache (the output would
@@ -1207,14 +1210,14 @@ namespace ppf
             if (out.out_mask & (1u << f))
               add(out.dst[f]);
           }
-        const int64_t footprint = int64_t(n_arrays) * g.n_nodes * int64_t(sizeof(number));
+        const int64_t footprint = int64_t(n_arrays) * g.n_nodes * int64_t(sizeof(ST));
         a.nt_store = stencil_store_policy == StorePolicy::NonTemporal ||
                      (stencil_store_policy == StorePolicy::Auto &&
                       footprint > stencil_nt_min_bytes);
       }
       // A pending coarse-grid solve takes this operator (src = p, dst = ap) and runs the
       // whole CG in one workgroup instead of one launch per vector operation.
-      if constexpr (std::is_same_v<number, double>)
+      if constexpr (std::is_same_v<number, double> && std::is_same_v<ST, double>)
         if (coarse_pending && lhs)
           {
             coarse_pending = false;
@@ -1303,7 +1306,7 @@ namespace ppf
       if (out.dirichlet_write == dirichlet_value)
         for (int f = 0; f < PPF_MAX_FIELDS; ++f)
           faces_follow = faces_follow || ((out.out_mask & (1u << f)) && out.dirichlet_mask[f]);
-      if constexpr (std::is_same_v<number, double>)
+      if constexpr (std::is_same_v<ST, double>)
         if (probe_req.pending && probe_req.block == b && !faces_follow && a.n_blocks > 0)
           {
             a.probe_result = probe_req.result;
@@ -1322,11 +1325,11 @@ namespace ppf
         constexpr int NS = decltype(ops_c)::value;
         if (lhs)
           hipLaunchKernelGGL(
-            (stencil_operator_kernel<PDE<number>, dim, degree, number, true, id, App::n_fields, WV, RR, NS>),
+            (stencil_operator_kernel<PDE<number>, dim, degree, number, true, id, App::n_fields, WV, RR, NS, ST>),
             grid, threads, 0, stream, pde, a, out, timer);
         else
           hipLaunchKernelGGL(
-            (stencil_operator_kernel<PDE<number>, dim, degree, number, false, id, App::n_fields, WV, RR, NS>),
+            (stencil_operator_kernel<PDE<number>, dim, degree, number, false, id, App::n_fields, WV, RR, NS, ST>),
             grid, threads, 0, stream, pde, a, out, timer);
       };
       using I1 = std::integral_constant<int, 1>;
@@ -1535,6 +1538,8 @@ namespace ppf
     void
     solve_explicit_fp32(int b)
     {
+      if (!explicit_fp32_copies && solve_explicit_mixed(b))
+        return;
       const auto        &blk = blocks[b];
       const int64_t      n   = fine.n_nodes;
       LevelFields<float> lf {};
@@ -1574,7 +1579,71 @@ namespace ppf
       apply_operator<float>(b, false, lf, {}, dst, scale_invm, dirichlet_value, false,
                             &dirichlet_values_f);
       for (auto f : blk.field_indices)
-        convert(field_ptr(int(f), 0), explicit_out_f[f].get(), n);
+        convert(field_ptr(int(f), 0), explicit_out_f[f].get(), n, enqueue_halt);
+    }
+
+    /// The FP32 explicit update straight on the FP64 state: the tiled stencil kernel,
+    /// instantiated with FP32 arithmetic, loads the FP64 inputs, rounds them in
+    /// registers (as the FP32 copies would be rounded) and stores the FP64 outputs,
+    /// and the constraint values are written in FP64.  Taken when the block's FP32 split
+    /// plan qualifies and, on the automatic path, has been checked against the generic
+    /// path on the fields (the first increment runs through the FP32 copies, which does
+    /// that check); false, with nothing launched, otherwise.
+    bool
+    solve_explicit_mixed(int b)
+    {
+      if (path == OperatorPath::Generic || path == OperatorPath::SplitNode)
+        return false;
+      const PlanEntry &entry = split_plan<float>(b, false);
+      if (!entry.plan.eligible || (path == OperatorPath::Auto && !entry.verified))
+        return false;
+      const prismspf::SolveBlock &blk = blocks[b];
+      const LevelFields<double>   lf  = fine_fields();
+      DepTable<double>            deps {};
+      OutTable<double>            out {};
+      for (std::size_t f = 0; f < fields.size(); ++f)
+        deps.dirichlet_mask[f] = dirichlet_mask(int(f));
+      for (const auto &in : declared_inputs(blk.dependencies_rhs))
+        {
+          const double *ptr = lf.ptr[in.field][in.slot];
+          if (!ptr)
+            return false;
+          deps.ptr[in.field][in.slot]   = ptr;
+          deps.flags[in.field][in.slot] = in.flags;
+        }
+      for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+        out.scratch_slot[f] = -1;
+      int k = 0;
+      for (auto fi : blk.field_indices)
+        {
+          const int f           = int(fi);
+          out.dst[f]            = field_ptr(f, 0);
+          out.dirichlet_mask[f] = deps.dirichlet_mask[f];
+          out.scratch_slot[f]   = k++;
+          out.out_mask         |= 1u << f;
+          if (out.dirichlet_mask[f])
+            {
+              if (!dirichlet_values[f].get())
+                throw Error(status_internal, "missing Dirichlet values");
+              out.dirichlet_values[f] = dirichlet_values[f].get();
+            }
+        }
+      out.scaling         = scale_invm;
+      out.dirichlet_write = dirichlet_value;
+      out.halt            = enqueue_halt;
+      bool launched       = false;
+      dispatch_block_id(typename App::BlockIds {}, blk.id, [&](auto bid) {
+        constexpr int id = decltype(bid)::value;
+        launched = launch_stencil<float, id, double>(b, false, fine, deps, out, entry.plan);
+      });
+      PPF_HIP_CHECK(hipGetLastError());
+      if (launched && !explicit_mixed_noted.count(b))
+        {
+          explicit_mixed_noted.insert(b);
+          log += "solve block " + std::to_string(blk.id) +
+                 " (rhs): FP32 stencil arithmetic on the FP64 state\n";
+        }
+      return launched;
     }
 
     // ------------------------------------------------------------- linear solve
@@ -3283,7 +3352,7 @@ namespace ppf
     speculable(unsigned int inc) const
     {
       if (inc == 0 || inputs.output_parameters.should_output(inc) || pde_d->get_user_stop() ||
-          explicit_fp32 || any_time_dependent_dirichlet())
+          any_time_dependent_dirichlet())
         return false;
       for (const auto &blk : blocks)
         {
@@ -3455,8 +3524,12 @@ namespace ppf
     /// Start every linear solve from zero instead of the previous solution (which the
     /// reference keeps as the next initial guess, linear_solver.h:330-336).
     bool                                   zero_initial_guess    = false;
-    /// FP32 operator arithmetic for explicit blocks (FP64 state), with its buffers.
+    /// FP32 operator arithmetic for explicit blocks (FP64 state), with its buffers;
+    /// explicit_fp32_copies: always through FP32 copies of the inputs and outputs (the
+    /// stencil kernel otherwise loads and stores the FP64 state itself).
     bool                                   explicit_fp32         = false;
+    bool                                   explicit_fp32_copies  = false;
+    std::set<int>                          explicit_mixed_noted;
     std::vector<DeviceBuffer<float>>       explicit_in_f, explicit_out_f, dirichlet_values_f;
 
     std::vector<std::vector<DeviceBuffer<double>>> storage;

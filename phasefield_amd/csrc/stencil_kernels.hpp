@@ -51,7 +51,10 @@ namespace ppf
   constexpr int stencil_tx      = 64; // tile width in x: one wavefront
   constexpr int stencil_max_ops = 2;
 
-  template <typename number>
+  /// Arguments of the stencil kernels: arithmetic in `number`, operand, value and
+  /// output arrays stored as ST (ST = double with number = float: FP64 state, FP32
+  /// arithmetic).
+  template <typename number, typename ST = number>
   struct StencilArgs
   {
     int    n[3];        // nodes per axis (n[2] = 1 for 2-D lattices)
@@ -80,11 +83,11 @@ namespace ppf
     double q_lower[3], q_step[3];
     double element_volume;
     int    n_ops;
-    const number *op[stencil_max_ops];
+    const ST     *op[stencil_max_ops];
     unsigned int  op_dmask[stencil_max_ops]; // Dirichlet faces read as zero (0: plain)
     /// Summed coefficient of operand k in output field f.
     number        beta[PPF_MAX_FIELDS][stencil_max_ops];
-    const number *val[PPF_MAX_FIELDS][PPF_N_SLOTS]; // value inputs of the hook
+    const ST     *val[PPF_MAX_FIELDS][PPF_N_SLOTS]; // value inputs of the hook
     unsigned int  val_dmask[PPF_MAX_FIELDS][PPF_N_SLOTS];
     /// Operand index k if the input is operand k (read from the ring), -1 otherwise.
     /// Every `op` and `val` pointer is valid (the host points unused entries at an
@@ -106,9 +109,9 @@ namespace ppf
     int           nt_store;
   };
 
-  template <typename number>
+  template <typename number, typename ST>
   __device__ void
-  stencil_probe(const StencilArgs<number> &a)
+  stencil_probe(const StencilArgs<number, ST> &a)
   {
     // Orders this thread's stores of node 0 before the reads below.
     __threadfence();
@@ -226,9 +229,9 @@ namespace ppf
 
   /// Operand s at lattice coordinate (x, y, z), which may lie a stencil radius outside
   /// the lattice (source node by wrap / mirror), zero on the operand's Dirichlet faces.
-  template <typename number>
+  template <typename number, typename ST>
   __device__ PPF_INLINE number
-  operand_at(const StencilArgs<number> &a, int s, int x, int y, int z)
+  operand_at(const StencilArgs<number, ST> &a, int s, int x, int y, int z)
   {
     const int xs = source_coord(x, a.n[0], a.periodic[0]);
     const int ys = source_coord(y, a.n[1], a.periodic[1]);
@@ -237,16 +240,16 @@ namespace ppf
                             face_bits(ys, a.n[1], a.periodic[1], 1) |
                             face_bits(zs, a.n[2], a.periodic[2], 2);
     const number v =
-      a.op[s][int64_t(xs) + int64_t(a.n[0]) * (int64_t(ys) + int64_t(a.n[1]) * zs)];
+      number(a.op[s][int64_t(xs) + int64_t(a.n[0]) * (int64_t(ys) + int64_t(a.n[1]) * zs)]);
     return (fb & a.op_dmask[s]) ? number(0) : v;
   }
 
   /// The operator at one node, evaluated directly from global memory (the remainder
   /// workgroups of stencil_operator_kernel; same arithmetic as the tiles).
   template <class PDE, int dim, int degree, typename number, bool lhs, int block_id, int NF,
-            int NS>
+            int NS, typename ST = number>
   __device__ void
-  stencil_node(const PDE &pde, const StencilArgs<number> &a, const OutTable<number> &out,
+  stencil_node(const PDE &pde, const StencilArgs<number, ST> &a, const OutTable<ST> &out,
                const prismspf::SimulationTimer &timer, int x, int y, int z)
   {
     constexpr int RAD = degree;
@@ -296,7 +299,7 @@ namespace ppf
             {
               const int fi = f < NF ? f : 0;
               const int al = a.val_alias[fi][s];
-              v = (fb & a.val_dmask[fi][s]) ? number(0) : a.val[fi][s][node];
+              v = (fb & a.val_dmask[fi][s]) ? number(0) : number(a.val[fi][s][node]);
               if (al >= 0)
                 v = operand_at(a, al < NS ? al : 0, x, y, z);
             }
@@ -318,16 +321,16 @@ namespace ppf
           acc += a.beta[f][k] * lap[k];
         const bool dir = (fb & out.dirichlet_mask[f]) != 0;
         if (!(dir && out.dirichlet_write == dirichlet_value))
-          out.dst[f][node] = dir ? number(0) : sc * acc;
+          out.dst[f][node] = dir ? ST(0) : ST(sc * acc);
       }
   }
 
   template <class PDE, int dim, int degree, typename number, bool lhs, int block_id, int NF,
-            int WV, int R, int NS>
+            int WV, int R, int NS, typename ST = number>
   __global__ void __launch_bounds__(stencil_tx * WV)
   stencil_operator_kernel(const PDE                       pde,
-                          const StencilArgs<number>       a,
-                          const OutTable<number>          out,
+                          const StencilArgs<number, ST>   a,
+                          const OutTable<ST>              out,
                           const prismspf::SimulationTimer timer)
   {
     constexpr int RAD = degree;
@@ -373,7 +376,8 @@ namespace ppf
           }
         else
           return;
-        stencil_node<PDE, dim, degree, number, lhs, block_id, NF, NS>(pde, a, out, timer, x, y, z);
+        stencil_node<PDE, dim, degree, number, lhs, block_id, NF, NS, ST>(pde, a, out, timer, x,
+                                                                          y, z);
         return;
       }
 
@@ -405,7 +409,7 @@ namespace ppf
     const int zb  = dim == 3 ? chk * a.z_chunk : 0; // even for Q2 (host)
     const int ze  = dim == 3 ? min(zb + a.z_chunk, n2) : 1;
     // Byte offsets fit 32 bits: the host runs this kernel on arrays below 4 GiB only.
-    const unsigned int plane_bytes = unsigned(n0) * unsigned(n1) * unsigned(sizeof(number));
+    const unsigned int plane_bytes = unsigned(n0) * unsigned(n1) * unsigned(sizeof(ST));
 
     const bool  nt = a.nt_store != 0;
     buffer_rsrc r_op[NS], r_val[NF][PPF_N_SLOTS], r_dst[NF];
@@ -445,7 +449,7 @@ namespace ppf
         yout[r]     = y < a.tile_n[1];
         ys[r]       = source_coord(y, n1, p1);
         fby[r]      = face_bits(ys[r], n1, p1, 1);
-        own_b[r]    = (unsigned(xs) + unsigned(n0) * unsigned(ys[r])) * unsigned(sizeof(number));
+        own_b[r]    = (unsigned(xs) + unsigned(n0) * unsigned(ys[r])) * unsigned(sizeof(ST));
         const bool py = degree == 2 && (ys[r] & 1);
         wy0[r]        = py ? a.tap[1][1][0] : a.tap[1][0][0];
         wy1[r]        = py ? a.tap[1][1][1] : a.tap[1][0][1];
@@ -470,9 +474,9 @@ namespace ppf
     const int          hx_y   = source_coord(y0 + hx_row, n1, p1);
     const unsigned int fb_hx  = face_bits(hx_x, n0, p0, 0) | face_bits(hx_y, n1, p1, 1);
     const unsigned int hy_b =
-      (unsigned(xs) + unsigned(n0) * unsigned(hy_y)) * unsigned(sizeof(number));
+      (unsigned(xs) + unsigned(n0) * unsigned(hy_y)) * unsigned(sizeof(ST));
     const unsigned int hx_b =
-      hx_on ? (unsigned(hx_x) + unsigned(n0) * unsigned(hx_y)) * unsigned(sizeof(number)) : hy_b;
+      hx_on ? (unsigned(hx_x) + unsigned(n0) * unsigned(hx_y)) * unsigned(sizeof(ST)) : hy_b;
     const int hy_ly = hy_on ? hy_row : LY;
     const int hx_ly = hx_on ? RAD + hx_row : LY;
     const int hx_lx = hx_on ? hx_col : tx;
@@ -481,9 +485,11 @@ namespace ppf
     const number sz_odd = a.sfac[2][0], sz_even = a.sfac[2][1], sz_bnd = a.sfac[2][2];
 
     // Register rings: slot of absolute plane p = (p - zb + RAD) mod S.
-    number       ring[NS][R][S];
-    number       ring_hy[NS][S], ring_hx[NS][S];
-    number       ring_v[S][R][NF][PPF_N_SLOTS];
+    // (stored as loaded; converted to the arithmetic type where they are used, so no
+    // load is waited for before its plane is reached)
+    ST           ring[NS][R][S];
+    ST           ring_hy[NS][S], ring_hx[NS][S];
+    ST           ring_v[S][R][NF][PPF_N_SLOTS];
     unsigned int ring_fb[S]; // face bits of the plane in the slot (uniform)
 
     // Source plane of z (wrap / mirror only outside [0, n2)).
@@ -498,15 +504,15 @@ namespace ppf
       for (int s = 0; s < NS; ++s)
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          ring[s][r][slot] = buffer_load<number>(r_op[s], own_b[r], po);
+          ring[s][r][slot] = buffer_load<ST>(r_op[s], own_b[r], po);
     };
     auto load_inputs = [&](int z, int slot) {
       const unsigned int po = unsigned(source_z(z)) * plane_bytes;
 #pragma unroll
       for (int s = 0; s < NS; ++s)
         {
-          ring_hy[s][slot] = buffer_load<number>(r_op[s], hy_b, po);
-          ring_hx[s][slot] = buffer_load<number>(r_op[s], hx_b, po);
+          ring_hy[s][slot] = buffer_load<ST>(r_op[s], hy_b, po);
+          ring_hx[s][slot] = buffer_load<ST>(r_op[s], hx_b, po);
         }
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -514,7 +520,7 @@ namespace ppf
         for (int f = 0; f < NF; ++f)
 #pragma unroll
           for (int s = 0; s < PPF_N_SLOTS; ++s)
-            ring_v[slot][r][f][s] = buffer_load<number>(r_val[f][s], own_b[r], po);
+            ring_v[slot][r][f][s] = buffer_load<ST>(r_val[f][s], own_b[r], po);
     };
 
     // Zeroes the operands of ring slot k on their Dirichlet faces.
@@ -523,7 +529,7 @@ namespace ppf
       for (int s = 0; s < NS; ++s)
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          ring[s][r][k] = ((fbx | fby[r] | ring_fb[k]) & a.op_dmask[s]) ? number(0)
+          ring[s][r][k] = ((fbx | fby[r] | ring_fb[k]) & a.op_dmask[s]) ? ST(0)
                                                                        : ring[s][r][k];
     };
 
@@ -557,10 +563,10 @@ namespace ppf
 #pragma unroll
           for (int r = 0; r < R; ++r)
             {
-              ctr[s][r] = ring[s][r][cs];
+              ctr[s][r] = number(ring[s][r][cs]);
               lds[buf][s][RAD + w + WV * r][RAD + tx] = ctr[s][r];
             }
-          number hyv = ring_hy[s][cs], hxv = ring_hx[s][cs];
+          number hyv = number(ring_hy[s][cs]), hxv = number(ring_hx[s][cs]);
           if constexpr (MASKED)
             {
               const unsigned int dm = a.op_dmask[s];
@@ -602,13 +608,15 @@ namespace ppf
               acc += wx1 * (L[ly][lx - 1] + L[ly][lx + 1]);
               acc += wy1[r] * (L[ly - 1][lx] + L[ly + 1][lx]);
               if constexpr (dim == 3)
-                acc += wz1 * (ring[s][r][(cs + S - 1) % S] + ring[s][r][(cs + 1) % S]);
+                acc += wz1 * (number(ring[s][r][(cs + S - 1) % S]) +
+                              number(ring[s][r][(cs + 1) % S]));
               if constexpr (RAD == 2)
                 {
                   acc += wx2 * (L[ly][lx - 2] + L[ly][lx + 2]);
                   acc += wy2[r] * (L[ly - 2][lx] + L[ly + 2][lx]);
                   if constexpr (dim == 3)
-                    acc += wz2 * (ring[s][r][(cs + S - 2) % S] + ring[s][r][(cs + 2) % S]);
+                    acc += wz2 * (number(ring[s][r][(cs + S - 2) % S]) +
+                                  number(ring[s][r][(cs + 2) % S]));
                 }
               lap[s] = acc;
             }
@@ -632,7 +640,7 @@ namespace ppf
                   {
                     const int fi = f < NF ? f : 0;
                     const int al = a.val_alias[fi][s];
-                    v            = ring_v[cs][r][fi][s];
+                    v            = number(ring_v[cs][r][fi][s]);
                     if constexpr (MASKED)
                       v = (fb & a.val_dmask[fi][s]) ? number(0) : v;
                     if (al == 0)
@@ -665,10 +673,10 @@ namespace ppf
                   // dirichlet_faces_kernel, which writes the constraint values.
                   const bool dir = (fb & out.dirichlet_mask[f]) != 0;
                   if (valid && !(dir && out.dirichlet_write == dirichlet_value))
-                    buffer_store(dir ? number(0) : scale * acc, r_dst[f], own_b[r], zoff, nt);
+                    buffer_store(dir ? ST(0) : ST(scale * acc), r_dst[f], own_b[r], zoff, nt);
                 }
               else if (valid)
-                buffer_store(scale * acc, r_dst[f], own_b[r], zoff, nt);
+                buffer_store(ST(scale * acc), r_dst[f], own_b[r], zoff, nt);
             }
           // Node 0 (lane 0, wave 0, row 0, plane 0 of workgroup 0, which owns tile 0 of
           // chunk 0) closes the increment's NaN probe.

@@ -95,10 +95,13 @@ namespace ppf
       x[i] = value;
   }
 
+  /// dst = src converted; a no-op once *halt is set (halt may be null).
   template <typename T, typename U>
   __global__ void
-  convert_kernel(T *__restrict__ dst, const U *__restrict__ src, int64_t n)
+  convert_kernel(T *__restrict__ dst, const U *__restrict__ src, int64_t n, const int *halt)
   {
+    if (halt && *reinterpret_cast<const volatile int *>(halt))
+      return;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n)
       dst[i] = T(src[i]);

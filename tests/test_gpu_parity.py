@@ -320,6 +320,38 @@ def test_fp32_operator_fp64_state_error(app, text, steps):
     assert errs["single"] > errs["double"]
 
 
+@pytest.mark.parametrize("app,text,steps", FP32_EXPLICIT_CASES, ids=[c[0] for c in FP32_EXPLICIT_CASES])
+def test_fp32_stencil_on_fp64_state_matches_fp32_copies(app, text, steps):
+    """explicit_precision = single runs the stencil kernel with FP32 arithmetic on the
+    FP64 state (loads rounded to FP32 in registers, FP64 stores) after a first increment
+    through FP32 copies; single_copies rounds the inputs into FP32 copies every
+    increment.  Both round every input once and evaluate the same FP32 arithmetic, so
+    the states agree to a few FP32 roundings (bound 1e-6 of the field's size; the FP32
+    path's own error is ~1e-6 .. 1e-5), the log names the block that took the kernel, and
+    the result keeps the single-precision error bound against the FP64 oracle."""
+    orc = Oracle(app, text)
+    assert orc.run(steps) == 0
+    res, logs = {}, {}
+    for mode in ("single", "single_copies"):
+        ctx = Context(app, text)
+        ctx.set_option("explicit_precision", mode)
+        ctx.set_option("stencil_min_nodes", "0")
+        assert ctx.run(steps) == 0
+        res[mode] = [ctx.get_field(f, AGE_LATEST).ravel().copy() for f in range(ctx.info.n_fields)]
+        logs[mode] = ctx.output_log()
+        ctx.close()
+    print(logs["single"], flush=True)
+    assert "FP32 stencil arithmetic on the FP64 state" in logs["single"]
+    assert "FP32 stencil arithmetic" not in logs["single_copies"]
+    for f, (a, b) in enumerate(zip(res["single"], res["single_copies"])):
+        d = float(np.abs(a - b).max())
+        scale = float(np.abs(b).max())
+        print(app, "field", f, "single vs single_copies max diff", d, "exact", d == 0.0,
+              flush=True)
+        assert d <= 1e-6 * scale
+        assert rel_err(a, orc.get_field(f, AGE_LATEST)) <= 1e-4
+
+
 LINEAR_CASES = [
     ("poisson_2d_q2", dict(dim=2, refinement=4, preconditioner="None", mg_depth=1)),
     ("poisson_2d_q2", dict(dim=2, refinement=4, preconditioner="Chebyshev", mg_depth=1)),
