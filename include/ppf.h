@@ -124,7 +124,11 @@ int ppf_field_name(ppf_ctx *ctx, int field, char *buf, int len);
  * state: the stencil kernel rounds its FP64 loads to FP32 in registers and stores FP64,
  * after a first increment through FP32 copies that checks the split operator; blocks
  * the stencil kernel does not take, and single_copies always, compute from FP32 copies
- * of their inputs and convert the results into the FP64 state). */
+ * of their inputs and convert the results into the FP64 state); "explicit_pairs" = 0|1
+ * (1: two consecutive explicit 3-D blocks, the second reading the first's output through
+ * its stencil, in one launch, with results identical to the two launches; off by
+ * default, being slower than the two launches), "pair_rows" = 2|3 and "pair_z_chunk" =
+ * <n> (its tile height and z chunk, 0: automatic). */
 int ppf_set_option(ppf_ctx *ctx, const char *key, const char *value);
 
 /* Field state `age` (0 = current, k = old_k, PPF_AGE_LATEST = the newest values) from/to

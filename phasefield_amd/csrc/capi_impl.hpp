@@ -418,6 +418,20 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
         for (auto &b : c.rt.blocks)
           b.linear_solver_parameters.max_iterations = unsigned(std::stoul(v));
       }
+    else if (k == "explicit_pairs")
+      {
+        if (v != "0" && v != "1")
+          throw ppf::Error(ppf::status_invalid, "explicit_pairs must be 0|1");
+        c.rt.explicit_pairs = v == "1";
+      }
+    else if (k == "pair_rows")
+      {
+        if (v != "2" && v != "3")
+          throw ppf::Error(ppf::status_invalid, "pair_rows must be 2|3");
+        c.rt.pair_rows = std::stoi(v);
+      }
+    else if (k == "pair_z_chunk")
+      c.rt.pair_z_chunk = std::max(0, std::stoi(v));
     else if (k == "explicit_precision")
       {
         if (v != "double" && v != "single" && v != "single_copies")

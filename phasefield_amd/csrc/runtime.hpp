@@ -14,6 +14,7 @@
 #include "grid.hpp"
 #include "operator_kernels.hpp"
 #include "operator_plan.hpp"
+#include "pair_kernels.hpp"
 #include "stencil_kernels.hpp"
 #include "vector_kernels.hpp"
 #include "vtu_writer.hpp"
@@ -92,6 +93,14 @@ namespace ppf
     if (!found)
       throw Error(status_invalid, "solve block id " + std::to_string(id) +
                                     " is not compiled into this application library");
+  }
+
+  /// Whether id is one of the compiled solve block ids.
+  template <int id, int... ids>
+  constexpr bool
+  has_block_id(std::integer_sequence<int, ids...>)
+  {
+    return ((id == ids) || ...);
   }
 
   /// Tridiagonal symmetric eigenvalues by bisection (Sturm sequences); used for the
@@ -1582,25 +1591,14 @@ namespace ppf
         convert(field_ptr(int(f), 0), explicit_out_f[f].get(), n, enqueue_halt);
     }
 
-    /// The FP32 explicit update straight on the FP64 state: the tiled stencil kernel,
-    /// instantiated with FP32 arithmetic, loads the FP64 inputs, rounds them in
-    /// registers (as the FP32 copies would be rounded) and stores the FP64 outputs,
-    /// and the constraint values are written in FP64.  Taken when the block's FP32 split
-    /// plan qualifies and, on the automatic path, has been checked against the generic
-    /// path on the fields (the first increment runs through the FP32 copies, which does
-    /// that check); false, with nothing launched, otherwise.
+    /// Dependency and output tables of the explicit update of block b on the FP64 state:
+    /// the rhs inputs from the field storage, the outputs into the current slots, M^-1
+    /// scaling and Dirichlet values; false if an input has no storage.
     bool
-    solve_explicit_mixed(int b)
+    explicit_tables(int b, DepTable<double> &deps, OutTable<double> &out)
     {
-      if (path == OperatorPath::Generic || path == OperatorPath::SplitNode)
-        return false;
-      const PlanEntry &entry = split_plan<float>(b, false);
-      if (!entry.plan.eligible || (path == OperatorPath::Auto && !entry.verified))
-        return false;
       const prismspf::SolveBlock &blk = blocks[b];
       const LevelFields<double>   lf  = fine_fields();
-      DepTable<double>            deps {};
-      OutTable<double>            out {};
       for (std::size_t f = 0; f < fields.size(); ++f)
         deps.dirichlet_mask[f] = dirichlet_mask(int(f));
       for (const auto &in : declared_inputs(blk.dependencies_rhs))
@@ -1631,7 +1629,252 @@ namespace ppf
       out.scaling         = scale_invm;
       out.dirichlet_write = dirichlet_value;
       out.halt            = enqueue_halt;
-      bool launched       = false;
+      return true;
+    }
+
+    /// Explicit blocks ba and bb (consecutive in the solve order) in one launch
+    /// (pair_kernel): bb's stencil operand is ba's output.  Taken for 3-D FP64 updates
+    /// when both blocks update one field each through split plans that have been checked
+    /// against the generic path (the first increment runs the blocks one by one, which
+    /// does that check), ba has one stencil operand, no field involved has Dirichlet
+    /// faces, and neither block reads what the other writes except bb reading ba's
+    /// output; false, with nothing launched, otherwise.
+    /// A pair that does not qualify; with PPF_VERBOSE set, says where (once per place).
+    bool
+    pair_skip(int ba, int line)
+    {
+      if (std::getenv("PPF_VERBOSE") && !explicit_pair_skipped.count(line))
+        {
+          explicit_pair_skipped.insert(line);
+          std::fprintf(stderr, "explicit pair at block %d: not fused (runtime.hpp:%d)\n",
+                       blocks[ba].id, line);
+        }
+      return false;
+    }
+
+    bool
+    solve_explicit_pair(int ba, int bb)
+    {
+      if constexpr (dim != 3)
+        return pair_skip(ba, __LINE__);
+      else
+        {
+          if (!explicit_pairs || explicit_fp32 || timer.get_increment() == 0)
+            return pair_skip(ba, __LINE__);
+          if (path == OperatorPath::Generic || path == OperatorPath::SplitNode)
+            return pair_skip(ba, __LINE__);
+          const prismspf::SolveBlock &A = blocks[ba], &B = blocks[bb];
+          if (A.solve_type != prismspf::Explicit || B.solve_type != prismspf::Explicit ||
+              A.field_indices.size() != 1 || B.field_indices.size() != 1)
+            return pair_skip(ba, __LINE__);
+          const int fa = int(*A.field_indices.begin()), fb = int(*B.field_indices.begin());
+          if (fa == fb || B.id != A.id + 1)
+            return pair_skip(ba, __LINE__);
+          const PlanEntry ea = split_plan<double>(ba, false);
+          const PlanEntry eb = split_plan<double>(bb, false);
+          if (!ea.plan.eligible || !eb.plan.eligible)
+            return pair_skip(ba, __LINE__);
+          if (path == OperatorPath::Auto && (!ea.verified || !eb.verified))
+            return pair_skip(ba, __LINE__);
+          DepTable<double> da {}, db {};
+          OutTable<double> oa {}, ob {};
+          if (!explicit_tables(ba, da, oa) || !explicit_tables(bb, db, ob))
+            return pair_skip(ba, __LINE__);
+          const double *dst_a = oa.dst[fa];
+          double       *dst_b = ob.dst[fb];
+          // One stencil operand per block; bb's is ba's output.
+          auto operand = [](const SplitPlan &pl, int out_f, const DepTable<double> &d,
+                            const double *&ptr, double &beta) {
+            if (pl.terms.n_terms < 1)
+              return false;
+            const SplitTerm &t0 = pl.terms.term[0];
+            beta                = 0.0;
+            for (int t = 0; t < pl.terms.n_terms; ++t)
+              {
+                const SplitTerm &tm = pl.terms.term[t];
+                if (tm.out_field != out_f || tm.in_field != t0.in_field ||
+                    tm.in_slot != t0.in_slot)
+                  return false;
+                beta += tm.beta;
+              }
+            ptr = d.ptr[t0.in_field][t0.in_slot];
+            return ptr != nullptr;
+          };
+          const double *op_a = nullptr, *op_b = nullptr;
+          double        beta_a = 0.0, beta_b = 0.0;
+          if (!operand(ea.plan, fa, da, op_a, beta_a) || !operand(eb.plan, fb, db, op_b, beta_b))
+            return pair_skip(ba, __LINE__);
+          if (op_b != dst_a || op_a == dst_a || op_a == dst_b)
+            return pair_skip(ba, __LINE__);
+          // No Dirichlet faces on the outputs or on anything read.
+          if (oa.dirichlet_mask[fa] || ob.dirichlet_mask[fb])
+            return pair_skip(ba, __LINE__);
+          for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+            for (int s = 0; s < PPF_N_SLOTS; ++s)
+              if ((da.ptr[f][s] || db.ptr[f][s]) && da.dirichlet_mask[f])
+                return pair_skip(ba, __LINE__);
+
+          PairArgs<double> a {};
+          for (int d = 0; d < 3; ++d)
+            {
+              a.n[d]        = fine.nodes[d];
+              a.periodic[d] = fine.periodic[d];
+              // grown-tile positions reach 2p beyond the lattice: one wrap / mirror
+              if (a.n[d] < 2 * degree + 1)
+                return pair_skip(ba, __LINE__);
+              const double tq2[2][3] = {{14.0, -8.0, 1.0}, {8.0, -4.0, 0.0}};
+              const double tq1[3]    = {2.0, -1.0, 0.0};
+              for (int par = 0; par < 2; ++par)
+                for (int k = 0; k < 3; ++k)
+                  a.tap[d][par][k] =
+                    (degree == 2 ? tq2[par][k] : tq1[k]) / (fine.h[d] * fine.h[d]);
+              for (int c = 0; c < 3; ++c)
+                a.sfac[d][c] = 1.0; // M^-1 scaling
+              a.q_lower[d] = fine.lower[d];
+              a.q_step[d]  = fine.h[d] / degree;
+            }
+          a.element_volume = fine.h[0] * fine.h[1] * fine.h[2];
+          if (int64_t(a.n[0]) * a.n[1] * a.n[2] * int64_t(sizeof(double)) > (int64_t(1) << 32))
+            return pair_skip(ba, __LINE__);
+          a.op_a   = op_a;
+          a.beta_a = beta_a;
+          a.fa     = fa;
+          a.dst_a  = const_cast<double *>(dst_a);
+          a.beta_b = beta_b;
+          a.fb     = fb;
+          a.dst_b  = dst_b;
+          for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+            for (int s = 0; s < PPF_N_SLOTS; ++s)
+              {
+                // Entries without a value input read nothing (alias 0).
+                a.alias_a[f][s] = 0;
+                a.alias_b[f][s] = 0;
+                a.val_a[f][s]   = op_a;
+                a.val_b[f][s]   = op_a;
+                if ((da.flags[f][s] & slot_values) && da.ptr[f][s] != op_a)
+                  {
+                    if (da.ptr[f][s] == dst_a || da.ptr[f][s] == dst_b)
+                      return pair_skip(ba, __LINE__);
+                    a.alias_a[f][s] = -1;
+                    a.val_a[f][s]   = da.ptr[f][s];
+                  }
+                if ((db.flags[f][s] & slot_values) && db.ptr[f][s] != dst_a)
+                  {
+                    if (db.ptr[f][s] == dst_b)
+                      return pair_skip(ba, __LINE__);
+                    a.alias_b[f][s] = -1;
+                    a.val_b[f][s]   = db.ptr[f][s];
+                  }
+              }
+          a.halt = enqueue_halt;
+
+          // Tiles: 64 columns (the last few columns node by node when they are at most
+          // 8), TY = 4 BR rows (the last tile row partial), z chunks for ~2 workgroups
+          // per compute unit.
+          const int br = pair_rows;
+          const int ty = pair_main_waves * br;
+          const int rx = a.n[0] % stencil_tx;
+          a.tile_n0    = (rx != 0 && rx <= 8 && a.n[0] >= 2 * stencil_tx) ? a.n[0] - rx : a.n[0];
+          a.tiles_x    = (a.tile_n0 + stencil_tx - 1) / stencil_tx;
+          a.tiles_y    = (a.n[1] + ty - 1) / ty;
+          a.rem        = int64_t(a.n[0] - a.tile_n0) * a.n[1] * a.n[2];
+          int zc       = pair_z_chunk;
+          if (zc <= 0)
+            {
+              const int64_t xy     = int64_t(a.tiles_x) * a.tiles_y;
+              const int64_t target = 2 * device_cus();
+              zc = int((int64_t(a.n[2]) * xy + target - 1) / target);
+              zc = std::max(zc, 4 * degree);
+            }
+          a.z_chunk  = std::min(zc, a.n[2]);
+          a.chunks_z = (a.n[2] + a.z_chunk - 1) / a.z_chunk;
+          const int64_t nblocks = int64_t(a.tiles_x) * a.tiles_y * a.chunks_z;
+          const int64_t nrem    = (a.rem + 255) / 256;
+          if (nblocks >= (int64_t(1) << 31) || nrem >= (int64_t(1) << 31))
+            return pair_skip(ba, __LINE__);
+          a.n_blocks = int(nblocks);
+          {
+            const int64_t footprint = 4 * fine.n_nodes * int64_t(sizeof(double));
+            a.nt_store = stencil_store_policy == StorePolicy::NonTemporal ||
+                         (stencil_store_policy == StorePolicy::Auto &&
+                          footprint > stencil_nt_min_bytes);
+          }
+          if (probe_req.pending && probe_req.block == bb && a.n_blocks > 0)
+            {
+              a.probe_result = probe_req.result;
+              a.probe_halt   = probe_req.halt;
+              a.probe_tag    = probe_req.tag;
+              a.probe_n      = probe_req.fields.n;
+              for (int f = 0; f < PPF_MAX_FIELDS; ++f)
+                a.probe_ptr[f] = probe_req.fields.ptr[f];
+              probe_req.pending = false;
+              probe_req.taken   = true;
+            }
+          bool launched = false;
+          dispatch_block_id(typename App::BlockIds {}, A.id, [&](auto aid) {
+            constexpr int id_a = decltype(aid)::value;
+            if constexpr (has_block_id<id_a + 1>(typename App::BlockIds {}))
+              if (B.id == id_a + 1)
+                {
+                  const dim3 grid {unsigned(nblocks)}, threads {stencil_tx, pair_waves};
+                  const auto &pde = *pde_of<double>();
+                  if (nrem > 0)
+                    hipLaunchKernelGGL((pair_node_kernel<PDE<double>, degree, double, id_a, id_a + 1, App::n_fields>),
+                                       dim3(unsigned(nrem)), dim3(256), 0, stream, pde, a, timer);
+                  if (br == 2)
+                    hipLaunchKernelGGL((pair_kernel<PDE<double>, degree, double, id_a, id_a + 1, App::n_fields, 2>),
+                                       grid, threads, 0, stream, pde, a, timer);
+                  else
+                    hipLaunchKernelGGL((pair_kernel<PDE<double>, degree, double, id_a, id_a + 1, App::n_fields, 3>),
+                                       grid, threads, 0, stream, pde, a, timer);
+                  launched = true;
+                }
+          });
+          if (!launched)
+            {
+              if (a.probe_result)
+                {
+                  probe_req.pending = true;
+                  probe_req.taken   = false;
+                }
+              return pair_skip(ba, __LINE__);
+            }
+          PPF_HIP_CHECK(hipGetLastError());
+          if (!explicit_pair_noted.count(ba))
+            {
+              explicit_pair_noted.insert(ba);
+              log += "solve blocks " + std::to_string(A.id) + " and " + std::to_string(B.id) +
+                     " (rhs): one launch\n";
+            }
+          for (auto f : A.field_indices)
+            rotated[f] = false;
+          for (auto f : B.field_indices)
+            rotated[f] = false;
+          return true;
+        }
+    }
+
+    /// The FP32 explicit update straight on the FP64 state: the tiled stencil kernel,
+    /// instantiated with FP32 arithmetic, loads the FP64 inputs, rounds them in
+    /// registers (as the FP32 copies would be rounded) and stores the FP64 outputs,
+    /// and the constraint values are written in FP64.  Taken when the block's FP32 split
+    /// plan qualifies and, on the automatic path, has been checked against the generic
+    /// path on the fields (the first increment runs through the FP32 copies, which does
+    /// that check); false, with nothing launched, otherwise.
+    bool
+    solve_explicit_mixed(int b)
+    {
+      if (path == OperatorPath::Generic || path == OperatorPath::SplitNode)
+        return false;
+      const PlanEntry &entry = split_plan<float>(b, false);
+      if (!entry.plan.eligible || (path == OperatorPath::Auto && !entry.verified))
+        return false;
+      const prismspf::SolveBlock &blk = blocks[b];
+      DepTable<double>            deps {};
+      OutTable<double>            out {};
+      if (!explicit_tables(b, deps, out))
+        return false;
+      bool launched = false;
       dispatch_block_id(typename App::BlockIds {}, blk.id, [&](auto bid) {
         constexpr int id = decltype(bid)::value;
         launched = launch_stencil<float, id, double>(b, false, fine, deps, out, entry.plan);
@@ -3282,8 +3525,15 @@ namespace ppf
       probe_req.pending = !solve.empty() && inc > 0 &&
                           blocks[solve.back()].solve_type == prismspf::Explicit;
       enqueue_halt = halt ? halt + ((inc + 1u) & 1u) : nullptr;
-      for (int b : solve)
-        solve_block(b);
+      for (std::size_t i = 0; i < solve.size(); ++i)
+        {
+          if (i + 1 < solve.size() && solve_explicit_pair(solve[i], solve[i + 1]))
+            {
+              ++i;
+              continue;
+            }
+          solve_block(solve[i]);
+        }
       enqueue_halt = nullptr;
       if (!probe_req.taken)
         {
@@ -3530,6 +3780,15 @@ namespace ppf
     bool                                   explicit_fp32         = false;
     bool                                   explicit_fp32_copies  = false;
     std::set<int>                          explicit_mixed_noted;
+    /// Consecutive explicit blocks in one launch (solve_explicit_pair): on/off (off by
+    /// default: at 128^3 Q2 the fused launch takes 45 us against 2 x 13 us for the two
+    /// stencil launches; its 5-wave workgroups hold 230 registers a lane, one
+    /// workgroup per compute unit), B rows per main wave of a tile (2 or 3), z chunk
+    /// (0: automatic).
+    bool                                   explicit_pairs = false;
+    int                                    pair_rows      = 3;
+    int                                    pair_z_chunk   = 0;
+    std::set<int>                          explicit_pair_noted, explicit_pair_skipped;
     std::vector<DeviceBuffer<float>>       explicit_in_f, explicit_out_f, dirichlet_values_f;
 
     std::vector<std::vector<DeviceBuffer<double>>> storage;

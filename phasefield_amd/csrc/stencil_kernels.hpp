@@ -254,7 +254,7 @@ namespace ppf
   {
     constexpr int RAD = degree;
     const int     c[3] = {x, y, z};
-    number        tap[3][3], sc = number(1);
+    number        tap[3][3], sf[3];
     unsigned int  fb = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -263,22 +263,24 @@ namespace ppf
         for (int k = 0; k < 3; ++k)
           tap[d][k] = odd ? a.tap[d][1][k] : a.tap[d][0][k];
         const int cl = scale_class<degree>(c[d], a.n[d], a.periodic[d]);
-        sc *= cl == 0 ? a.sfac[d][0] : (cl == 1 ? a.sfac[d][1] : a.sfac[d][2]);
+        sf[d]        = cl == 0 ? a.sfac[d][0] : (cl == 1 ? a.sfac[d][1] : a.sfac[d][2]);
         fb |= face_bits(c[d], a.n[d], a.periodic[d], d);
       }
+    // Centre tap and scale factor associated as in the tiles: x + (y + z), s_x (s_y s_z).
+    const number sc = sf[0] * (sf[1] * (dim == 3 ? sf[2] : number(1)));
     number lap[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
       {
-        number acc = (tap[0][0] + tap[1][0] + (dim == 3 ? tap[2][0] : number(0))) *
+        number acc = (tap[0][0] + (tap[1][0] + (dim == 3 ? tap[2][0] : number(0)))) *
                      operand_at(a, s, x, y, z);
 #pragma unroll
         for (int o = 1; o <= RAD; ++o)
           {
-            acc += tap[0][o] * (operand_at(a, s, x - o, y, z) + operand_at(a, s, x + o, y, z));
-            acc += tap[1][o] * (operand_at(a, s, x, y - o, z) + operand_at(a, s, x, y + o, z));
+            acc = fma(tap[0][o], operand_at(a, s, x - o, y, z) + operand_at(a, s, x + o, y, z), acc);
+            acc = fma(tap[1][o], operand_at(a, s, x, y - o, z) + operand_at(a, s, x, y + o, z), acc);
             if constexpr (dim == 3)
-              acc += tap[2][o] * (operand_at(a, s, x, y, z - o) + operand_at(a, s, x, y, z + o));
+              acc = fma(tap[2][o], operand_at(a, s, x, y, z - o) + operand_at(a, s, x, y, z + o), acc);
           }
         lap[s] = acc;
       }
@@ -318,7 +320,7 @@ namespace ppf
         number acc = (fc.value_term_mask & (1u << f)) ? fc.value_term[f] : number(0);
 #pragma unroll
         for (int k = 0; k < NS; ++k)
-          acc += a.beta[f][k] * lap[k];
+          acc = fma(a.beta[f][k], lap[k], acc);
         const bool dir = (fb & out.dirichlet_mask[f]) != 0;
         if (!(dir && out.dirichlet_write == dirichlet_value))
           out.dst[f][node] = dir ? ST(0) : ST(sc * acc);
@@ -605,18 +607,18 @@ namespace ppf
             {
               const number(&L)[LY + 1][LX] = lds[buf][s];
               number acc = c0 * ctr[s][r];
-              acc += wx1 * (L[ly][lx - 1] + L[ly][lx + 1]);
-              acc += wy1[r] * (L[ly - 1][lx] + L[ly + 1][lx]);
+              acc = fma(wx1, L[ly][lx - 1] + L[ly][lx + 1], acc);
+              acc = fma(wy1[r], L[ly - 1][lx] + L[ly + 1][lx], acc);
               if constexpr (dim == 3)
-                acc += wz1 * (number(ring[s][r][(cs + S - 1) % S]) +
-                              number(ring[s][r][(cs + 1) % S]));
+                acc = fma(wz1, number(ring[s][r][(cs + S - 1) % S]) +
+                              number(ring[s][r][(cs + 1) % S]), acc);
               if constexpr (RAD == 2)
                 {
-                  acc += wx2 * (L[ly][lx - 2] + L[ly][lx + 2]);
-                  acc += wy2[r] * (L[ly - 2][lx] + L[ly + 2][lx]);
+                  acc = fma(wx2, L[ly][lx - 2] + L[ly][lx + 2], acc);
+                  acc = fma(wy2[r], L[ly - 2][lx] + L[ly + 2][lx], acc);
                   if constexpr (dim == 3)
-                    acc += wz2 * (number(ring[s][r][(cs + S - 2) % S]) +
-                                  number(ring[s][r][(cs + 2) % S]));
+                    acc = fma(wz2, number(ring[s][r][(cs + S - 2) % S]) +
+                                  number(ring[s][r][(cs + 2) % S]), acc);
                 }
               lap[s] = acc;
             }
@@ -666,7 +668,7 @@ namespace ppf
               number acc = (fc.value_term_mask & (1u << f)) ? fc.value_term[f] : number(0);
 #pragma unroll
               for (int k = 0; k < NS; ++k)
-                acc += a.beta[f][k] * lap[k];
+                acc = fma(a.beta[f][k], lap[k], acc);
               if constexpr (MASKED)
                 {
                   // Dirichlet nodes: zero (operator rows dropped), or left to
