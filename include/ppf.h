@@ -128,7 +128,8 @@ int ppf_field_name(ppf_ctx *ctx, int field, char *buf, int len);
  * (1: two consecutive explicit 3-D blocks, the second reading the first's output through
  * its stencil, in one launch, with results identical to the two launches; off by
  * default, being slower than the two launches), "pair_rows" = 2|3 and "pair_z_chunk" =
- * <n> (its tile height and z chunk, 0: automatic). */
+ * <n> (its tile height and z chunk, 0: automatic); "pipeline_depth" = 1|2|3 (explicit
+ * increments ppf_run enqueues ahead of the oldest unread NaN probe, default 2). */
 int ppf_set_option(ppf_ctx *ctx, const char *key, const char *value);
 
 /* Field state `age` (0 = current, k = old_k, PPF_AGE_LATEST = the newest values) from/to

@@ -418,6 +418,13 @@ ppf_set_option(ppf_ctx *ctx, const char *key, const char *value)
         for (auto &b : c.rt.blocks)
           b.linear_solver_parameters.max_iterations = unsigned(std::stoul(v));
       }
+    else if (k == "pipeline_depth")
+      {
+        const int d = std::stoi(v);
+        if (d < 1 || d > ppf::max_pipeline_depth)
+          throw ppf::Error(ppf::status_invalid, "pipeline_depth must be 1, 2 or 3");
+        c.rt.pipeline_depth = d;
+      }
     else if (k == "explicit_pairs")
       {
         if (v != "0" && v != "1")

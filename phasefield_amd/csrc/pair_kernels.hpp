@@ -61,6 +61,7 @@ namespace ppf
     /// Entry-0 NaN probe of the increment (probe_result null: none), as in StencilArgs.
     int          *probe_result;
     int          *probe_halt;
+    unsigned int  probe_hmask;
     int           probe_tag;
     int           probe_n;
     const double *probe_ptr[PPF_MAX_FIELDS];
@@ -80,7 +81,9 @@ namespace ppf
       if (bad < 0 && !isfinite(*reinterpret_cast<const volatile double *>(a.probe_ptr[f])))
         bad = f;
     if (bad >= 0 && a.probe_halt)
-      *reinterpret_cast<volatile int *>(a.probe_halt) = 1;
+      for (int i = 0; i < 32; ++i)
+        if ((a.probe_hmask >> i) & 1u)
+          reinterpret_cast<volatile int *>(a.probe_halt)[i] = 1;
     *reinterpret_cast<volatile int *>(a.probe_result) = (a.probe_tag << 5) | (bad + 1);
   }
 

@@ -39,6 +39,7 @@
 #include <sstream>
 #include <string>
 #include <utility>
+#include <deque>
 #include <set>
 #include <vector>
 
@@ -94,6 +95,9 @@ namespace ppf
       throw Error(status_invalid, "solve block id " + std::to_string(id) +
                                     " is not compiled into this application library");
   }
+
+  /// Most increments the run loop enqueues ahead of the oldest unread NaN probe.
+  constexpr int max_pipeline_depth = 3;
 
   /// Whether id is one of the compiled solve block ids.
   template <int id, int... ids>
@@ -199,7 +203,7 @@ namespace ppf
       scalar_buf.resize(8);
       coarse_slots.resize(8);
       coarse_flag.resize(1);
-      halt_flag.resize(2);
+      halt_flag.resize(max_pipeline_depth + 1);
       setup_storage();
       compute_dirichlet_values(fine, dirichlet_values);
     }
@@ -1320,6 +1324,7 @@ namespace ppf
           {
             a.probe_result = probe_req.result;
             a.probe_halt   = probe_req.halt;
+            a.probe_hmask  = probe_req.hmask;
             a.probe_tag    = probe_req.tag;
             a.probe_n      = probe_req.fields.n;
             for (int f = 0; f < PPF_MAX_FIELDS; ++f)
@@ -1803,6 +1808,7 @@ namespace ppf
             {
               a.probe_result = probe_req.result;
               a.probe_halt   = probe_req.halt;
+              a.probe_hmask  = probe_req.hmask;
               a.probe_tag    = probe_req.tag;
               a.probe_n      = probe_req.fields.n;
               for (int f = 0; f < PPF_MAX_FIELDS; ++f)
@@ -3489,12 +3495,13 @@ namespace ppf
     /// rides on the last operator launch when that launch is the stencil kernel of an
     /// explicit block, else it is a kernel of its own.
     ///
-    /// With `halt` (two words) given, increment k's launches read halt[(k + 1) & 1],
-    /// which the probe of increment k - 1 sets on a non-finite entry, and do nothing
-    /// once it is set; increment k's probe sets halt[k & 1].  A probe that shares a
-    /// launch with its increment thus never stops that increment's own workgroups.
+    /// With `halt` (a ring of `ring` words) given, increment k's launches read word
+    /// halt[slot] (slot = k mod ring) and do nothing once it is set; increment k's probe
+    /// sets every other word on a non-finite entry, so the up to ring - 1 increments
+    /// enqueued after k become no-ops, and a probe that shares a launch with its
+    /// increment never stops that increment's own workgroups.
     void
-    enqueue_increment(bool do_output, int slot, int *halt)
+    enqueue_increment(bool do_output, int slot, int *halt, int ring = 2)
     {
       const unsigned int inc = timer.get_increment();
       if (any_time_dependent_dirichlet())
@@ -3513,18 +3520,18 @@ namespace ppf
       for (std::size_t f = 0; f < fields.size(); ++f)
         pf.ptr[f] = field_ptr(int(f), 0);
       probe_tag[slot] = int(++probe_count & 0x3ffffffu);
-      int *result     = probe_flag.get(2) + slot;
+      int *result     = probe_flag.get(max_pipeline_depth + 1) + slot;
       *reinterpret_cast<volatile int *>(result) = -1;
-      int *halt_out = halt ? halt + (inc & 1u) : nullptr;
       probe_req         = {};
       probe_req.result  = result;
-      probe_req.halt    = halt_out;
+      probe_req.halt    = halt;
+      probe_req.hmask   = halt ? ((1u << ring) - 1u) & ~(1u << slot) : 0u;
       probe_req.tag     = probe_tag[slot];
       probe_req.fields  = pf;
       probe_req.block   = solve.empty() ? -1 : solve.back();
       probe_req.pending = !solve.empty() && inc > 0 &&
                           blocks[solve.back()].solve_type == prismspf::Explicit;
-      enqueue_halt = halt ? halt + ((inc + 1u) & 1u) : nullptr;
+      enqueue_halt = halt ? halt + slot : nullptr;
       for (std::size_t i = 0; i < solve.size(); ++i)
         {
           if (i + 1 < solve.size() && solve_explicit_pair(solve[i], solve[i + 1]))
@@ -3538,7 +3545,7 @@ namespace ppf
       if (!probe_req.taken)
         {
           hipLaunchKernelGGL(finite_probe_fields_kernel, dim3(1), dim3(1), 0, stream, result,
-                             pf, halt_out, probe_tag[slot]);
+                             pf, halt, probe_req.hmask, probe_tag[slot]);
           PPF_HIP_CHECK(hipGetLastError());
         }
       probe_req = {};
@@ -3550,7 +3557,7 @@ namespace ppf
     bool
     probe_found_nan(int slot)
     {
-      const volatile int *word = probe_flag.get(2) + slot;
+      const volatile int *word = probe_flag.get(max_pipeline_depth + 1) + slot;
       const int           tag  = probe_tag[slot];
       for (unsigned long spin = 1;; ++spin)
         {
@@ -3627,22 +3634,29 @@ namespace ppf
     /// stopping after the first increment whose probe finds a non-finite entry
     /// (status 2) or on a user stop (3).
     ///
-    /// Runs of explicit increments without output are pipelined one deep: increment
-    /// k+1 is enqueued before increment k's probe is read, so the GPU never idles on
-    /// the host round trip.  Increment k+1's launches carry a device flag that
-    /// increment k's probe sets on a NaN, which makes them no-ops; the host then
-    /// undoes k+1's rotation and timer step.  Results, the log and the final state
-    /// are those of the one-at-a-time loop.
+    /// Runs of explicit increments without output are pipelined `pipeline_depth`
+    /// deep: increments k+1 .. k+depth are enqueued before increment k's probe is
+    /// read, so the GPU neither idles on the host round trip nor on a late host
+    /// thread.  Their launches carry device flags that increment k's probe sets on a
+    /// NaN, which makes them no-ops; the host then undoes their rotations and timer
+    /// steps.  Results, the log and the final state are those of the one-at-a-time
+    /// loop.
     int
     run(unsigned int n_increments)
     {
-      int  status = 0;
-      int *halt   = halt_flag.get();
-      hipLaunchKernelGGL(fill_kernel<int>, dim3(1), dim3(64), 0, stream, halt, 0, int64_t(2));
+      int       status = 0;
+      int      *halt   = halt_flag.get();
+      const int depth  = std::clamp(pipeline_depth, 1, max_pipeline_depth);
+      const int ring   = depth + 1;
+      hipLaunchKernelGGL(fill_kernel<int>, dim3(1), dim3(64), 0, stream, halt, 0,
+                         int64_t(ring));
       PPF_HIP_CHECK(hipGetLastError());
-      bool      pending = false; // an increment enqueued whose probe is unread
-      int       pending_slot = 0;
-      HostState pending_before {};
+      struct Pending
+      {
+        int       slot;
+        HostState before;
+      };
+      std::deque<Pending> pending; // enqueued increments whose probes are unread
       // NaN in the pending increment: back to the state before its rotation, log its
       // norms, then rotate and step the timer as the one-at-a-time loop does.
       auto settle_nan = [&](const HostState &before) {
@@ -3656,16 +3670,33 @@ namespace ppf
         timer.increment();
         return 2;
       };
+      // The oldest pending increment found a NaN: the increments enqueued after it did
+      // nothing on the device; undo their rotations and timer steps (newest first),
+      // then settle the NaN increment.
+      auto settle_oldest = [&](const Pending &p) {
+        for (auto it = pending.rbegin(); it != pending.rend(); ++it)
+          {
+            unrotate_all();
+            timer   = it->before.timer;
+            rotated = it->before.rotated;
+          }
+        pending.clear();
+        return settle_nan(p.before);
+      };
+      // Reads the oldest pending probe; nonzero status on a NaN.
+      auto read_oldest = [&]() {
+        const Pending p = pending.front();
+        pending.pop_front();
+        return probe_found_nan(p.slot) ? settle_oldest(p) : 0;
+      };
       while (status == 0 && timer.get_increment() <= n_increments)
         {
           const unsigned int inc = timer.get_increment();
           if (!speculable(inc))
             {
-              if (pending)
+              if (!pending.empty())
                 {
-                  pending = false;
-                  if (probe_found_nan(pending_slot))
-                    status = settle_nan(pending_before);
+                  status = read_oldest();
                   continue;
                 }
               status = solve_increment();
@@ -3673,26 +3704,16 @@ namespace ppf
               continue;
             }
           const HostState before {timer, rotated};
-          const int       slot = int(inc & 1u);
-          enqueue_increment(false, slot, halt);
+          const int       slot = int(inc % unsigned(ring));
+          enqueue_increment(false, slot, halt, ring);
           rotate_all();
           timer.increment();
-          if (pending && probe_found_nan(pending_slot))
-            {
-              // The increment just enqueued did nothing on the device.
-              unrotate_all();
-              timer   = before.timer;
-              rotated = before.rotated;
-              status  = settle_nan(pending_before);
-              pending = false;
-              break;
-            }
-          pending        = true;
-          pending_slot   = slot;
-          pending_before = before;
+          pending.push_back({slot, before});
+          if (int(pending.size()) > depth)
+            status = read_oldest();
         }
-      if (pending && probe_found_nan(pending_slot))
-        status = settle_nan(pending_before);
+      while (status == 0 && !pending.empty())
+        status = read_oldest();
       sync();
       return status;
     }
@@ -3821,14 +3842,17 @@ namespace ppf
     /// Largest coarse lattice solved by one workgroup.
     int64_t                                        coarse_fused_max_nodes = 4096;
     DeviceBuffer<int>                              coarse_flag;
-    PinnedBuffer<int>                              probe_flag; // two probe slots
-    int                                            probe_tag[2] = {0, 0};
+    PinnedBuffer<int>                              probe_flag; // probe slots (pipeline ring)
+    int                                            probe_tag[max_pipeline_depth + 1] = {};
+    /// Increments enqueued ahead of the oldest unread NaN probe (option pipeline_depth).
+    int                                            pipeline_depth = 2;
     unsigned int                                   probe_count  = 0;
     /// The NaN probe of the increment being enqueued, until a launch takes it.
     struct ProbeRequest
     {
       int        *result  = nullptr;
       int        *halt    = nullptr;
+      unsigned    hmask   = 0;
       int         tag     = 0;
       ProbeFields fields  = {};
       int         block   = -1;

@@ -99,9 +99,11 @@ namespace ppf
     /// Entry-0 NaN probe of an increment, fused into its last launch (probe_result
     /// null: none).  The thread that owns node 0 writes (tag << 5) | (bad + 1) to
     /// *probe_result once its own outputs are stored, bad being the first field whose
-    /// entry 0 is not finite (-1: none), and sets *probe_halt on a bad field.
+    /// entry 0 is not finite (-1: none), and on a bad field sets the words
+    /// probe_halt[i] of the bits i of probe_hmask.
     int          *probe_result;
     int          *probe_halt;
+    unsigned int  probe_hmask;
     int           probe_tag;
     int           probe_n;
     const double *probe_ptr[PPF_MAX_FIELDS];
@@ -120,7 +122,9 @@ namespace ppf
       if (bad < 0 && !isfinite(*reinterpret_cast<const volatile double *>(a.probe_ptr[f])))
         bad = f;
     if (bad >= 0 && a.probe_halt)
-      *reinterpret_cast<volatile int *>(a.probe_halt) = 1;
+      for (int i = 0; i < 32; ++i)
+        if ((a.probe_hmask >> i) & 1u)
+          reinterpret_cast<volatile int *>(a.probe_halt)[i] = 1;
     *reinterpret_cast<volatile int *>(a.probe_result) = (a.probe_tag << 5) | (bad + 1);
   }
 

@@ -620,7 +620,7 @@ namespace ppf
   /// the stream do nothing.
   __global__ void
   finite_probe_fields_kernel(int *__restrict__ result, const ProbeFields fields,
-                             int *__restrict__ halt, int tag)
+                             int *__restrict__ halt, unsigned int hmask, int tag)
   {
     if (threadIdx.x == 0 && blockIdx.x == 0)
       {
@@ -629,7 +629,9 @@ namespace ppf
           if (bad < 0 && !isfinite(fields.ptr[f][0]))
             bad = f;
         if (bad >= 0 && halt)
-          *halt = 1;
+          for (int i = 0; i < 32; ++i)
+            if ((hmask >> i) & 1u)
+              halt[i] = 1;
         *reinterpret_cast<volatile int *>(result) = (tag << 5) | (bad + 1);
       }
   }

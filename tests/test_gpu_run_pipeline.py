@@ -1,4 +1,5 @@
-"""The increment loop of ppf_run enqueues explicit increments one ahead of the NaN probe.
+"""The increment loop of ppf_run enqueues explicit increments up to pipeline_depth (1, 2
+by default, or 3) ahead of the NaN probe.
 
 Every case diverges (explicit steps far above the stability limit), so the run stops
 with status 2 part-way.  The pipelined ppf_run must leave exactly what a loop of
@@ -49,6 +50,13 @@ def state(ctx: Context):
 # FP32 operator arithmetic on the FP64 state (explicit_precision): the stencil kernel on
 # the FP64 arrays (single), and FP32 copies converted into the state (single_copies),
 # whose conversions also skip once the halt flag is set.
+DIVERGING_DEPTH = [
+    (DIVERGING[0][0], DIVERGING[0][1], {"pipeline_depth": "1"}),
+    (DIVERGING[0][0], DIVERGING[0][1], {"pipeline_depth": "3"}),
+    (DIVERGING[1][0], DIVERGING[1][1], {"pipeline_depth": "3"}),
+    (DIVERGING[3][0], DIVERGING[3][1], {"pipeline_depth": "1"}),
+    (DIVERGING[3][0], DIVERGING[3][1], {"pipeline_depth": "3"}),
+]
 DIVERGING_FP32 = [
     ("cahn_hilliard_3d_q2", DIVERGING[3][1], {"explicit_precision": "single", "stencil_min_nodes": "0"}),
     ("allen_cahn_2d_q1", DIVERGING[0][1], {"explicit_precision": "single", "stencil_min_nodes": "0"}),
@@ -63,8 +71,10 @@ def _context(app, text, opts):
     return ctx
 
 
-@pytest.mark.parametrize("app,text,opts", [c + ({},) for c in DIVERGING] + DIVERGING_FP32,
+@pytest.mark.parametrize("app,text,opts",
+                         [c + ({},) for c in DIVERGING] + DIVERGING_DEPTH + DIVERGING_FP32,
                          ids=[f"{c[0]}-{i}" for i, c in enumerate(DIVERGING)] +
+                             [f"{c[0]}-depth{c[2]['pipeline_depth']}-{i}" for i, c in enumerate(DIVERGING_DEPTH)] +
                              [f"{c[0]}-{c[2]['explicit_precision']}" for c in DIVERGING_FP32])
 def test_pipelined_run_stops_like_single_increments(app, text, opts):
     n = 40
@@ -80,7 +90,7 @@ def test_pipelined_run_stops_like_single_increments(app, text, opts):
     assert (ip.increment, ip.time) == (is_.increment, is_.time)
     assert piped.output_log() == single.output_log()
     assert state(piped) == state(single)
-    if opts:
+    if "explicit_precision" in opts:
         return  # FP32 rounding moves the divergence away from the FP64 oracle's
 
     orc = Oracle(app, text)
