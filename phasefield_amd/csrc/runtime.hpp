@@ -39,6 +39,7 @@
 #include <sstream>
 #include <string>
 #include <utility>
+#include <chrono>
 #include <deque>
 #include <set>
 #include <vector>
@@ -3705,16 +3706,22 @@ namespace ppf
             }
           const HostState before {timer, rotated};
           const int       slot = int(inc % unsigned(ring));
+          const auto      t0   = std::chrono::steady_clock::now();
           enqueue_increment(false, slot, halt, ring);
           rotate_all();
           timer.increment();
           pending.push_back({slot, before});
+          enqueue_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+          ++enqueue_count;
           if (int(pending.size()) > depth)
             status = read_oldest();
         }
       while (status == 0 && !pending.empty())
         status = read_oldest();
       sync();
+      if (std::getenv("PPF_VERBOSE") && enqueue_count)
+        std::fprintf(stderr, "run: %ld increments enqueued ahead, %.2f us of host time each\n",
+                     enqueue_count, 1e6 * enqueue_seconds / double(enqueue_count));
       return status;
     }
 
@@ -3846,6 +3853,9 @@ namespace ppf
     int                                            probe_tag[max_pipeline_depth + 1] = {};
     /// Increments enqueued ahead of the oldest unread NaN probe (option pipeline_depth).
     int                                            pipeline_depth = 2;
+    /// Host time spent enqueueing pipelined increments (reported with PPF_VERBOSE).
+    double                                         enqueue_seconds = 0.0;
+    long                                           enqueue_count   = 0;
     unsigned int                                   probe_count  = 0;
     /// The NaN probe of the increment being enqueued, until a launch takes it.
     struct ProbeRequest
