@@ -50,6 +50,11 @@ namespace ppf
 {
   constexpr int stencil_tx      = 64; // tile width in x: one wavefront
   constexpr int stencil_max_ops = 2;
+  /// Plane steps between the load of a stencil plane and its first use (odd: the ring
+  /// length stays even).  Measured at 128^3 Q2: 3 steps made the CH blocks 12.4 ->
+  /// 13.8 us and the FP64 Poisson apply 61 -> 70 us (more registers, and a longer
+  /// prologue per z chunk), so one step it is.
+  constexpr int stencil_lookahead = 1;
 
   /// Arguments of the stencil kernels: arithmetic in `number`, operand, value and
   /// output arrays stored as ST (ST = double with number = float: FP64 state, FP32
@@ -344,8 +349,10 @@ namespace ppf
     constexpr int TY  = WV * R;
     constexpr int LX  = stencil_tx + 2 * RAD;
     constexpr int LY  = TY + 2 * RAD;
-    constexpr int D   = RAD + 1; // load distance in planes
-    constexpr int S   = T + 1;   // ring length
+    constexpr int LA  = stencil_lookahead; // steps between a plane's load and its use
+    constexpr int D   = RAD + LA;          // load distance in planes
+    constexpr int S   = T + LA;            // ring length
+    static_assert(LA % 2 == 1, "the ring length must be even (z parity from the phase)");
     static_assert(WV >= 2 * RAD, "the y-halo rows are loaded by one wave each");
     __shared__ number lds[2][NS][LY + 1][LX]; // row LY: spare target of idle lanes
 
@@ -698,10 +705,10 @@ namespace ppf
           // Prologue: own planes zb-RAD .. zb+RAD (slots 0 .. 2 RAD), halo and value
           // inputs of planes zb .. zb+RAD (slots RAD .. 2 RAD).
 #pragma unroll
-          for (int k = 0; k < T; ++k)
+          for (int k = 0; k < T + LA - 1; ++k)
             load_own(min(zb - RAD + k, ze - 1 + RAD), k);
 #pragma unroll
-          for (int k = RAD; k < T; ++k)
+          for (int k = RAD; k < T + LA - 1; ++k)
             load_inputs(min(zb - RAD + k, ze - 1), k);
           if constexpr (MASKED)
             {
